@@ -1,0 +1,78 @@
+"""ctypes binding of libheu_paillier.so (the C ABI in include/heu_paillier.h).
+
+Loading fails loudly when the library has not been built: there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libheu_paillier.so")
+
+HEU_OK, HEU_EINVAL, HEU_ERANGE, HEU_ENOTINV, HEU_ENOMEM, HEU_EDEVICE, HEU_EUNSUPPORTED, HEU_EKEY = range(8)
+
+# Every entry point declared in include/heu_paillier.h, with its ctypes signature.
+P, U32, U64, SZ, I = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int
+SIGNATURES = {
+    "heu_last_error": (ctypes.c_char_p, []),
+    "heu_ctx_create": (I, [I, ctypes.POINTER(P)]),
+    "heu_ctx_destroy": (I, [P]),
+    "heu_ctx_set_stream": (I, [P, P]),
+    "heu_ctx_stream": (P, [P]),
+    "heu_ctx_sync": (I, [P]),
+    "heu_keygen": (I, [U32, ctypes.POINTER(U64), P, P, P, P]),
+    "heu_key_create": (I, [P, P, P, U32, P, P, U32, U32, U32, ctypes.POINTER(P)]),
+    "heu_key_destroy": (I, [P]),
+    "heu_key_info": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)]),
+    "heu_key_device_bytes": (SZ, [P]),
+    "heu_encrypt_i64": (I, [P, P, P, U32, SZ, P]),
+    "heu_encrypt": (I, [P, P, P, U32, P, U32, SZ, P]),
+    "heu_randomize": (I, [P, P, P, U32, SZ, P]),
+    "heu_decrypt": (I, [P, P, SZ, P, U32, P]),
+    "heu_add": (I, [P, P, P, SZ, P]),
+    "heu_add_i64": (I, [P, P, P, SZ, P]),
+    "heu_sub": (I, [P, P, P, SZ, P]),
+    "heu_negate": (I, [P, P, SZ, P]),
+    "heu_mul_i64": (I, [P, P, P, SZ, P]),
+    "heu_sum": (I, [P, P, P, SZ, P]),
+    "heu_dot_i64": (I, [P, P, P, P, SZ, P]),
+    "heu_to_mont": (I, [P, P, SZ, U32, P]),
+    "heu_from_mont": (I, [P, P, SZ, U32, P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class HeuError(RuntimeError):
+    """A failing engine call (the reference raises yacl::Exception, surfaced
+    to Python as RuntimeError)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[heu error {code}] {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m heu_amd.build` "
+                                   "(the engine has no CPU fallback)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != HEU_OK:
+        msg = lib().heu_last_error()
+        raise HeuError(rc, msg.decode() if msg else "")

@@ -1,0 +1,109 @@
+// kernels.hpp -- argument blocks shared by the host engine and the gfx950
+// kernels, and the per-key-size table of launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace heu_hip {
+
+typedef uint8_t u8;
+
+// Device error flags (OR-ed by kernels, read by the engine at sync).
+constexpr uint32_t kErrRange = 1u;          // an input outside the supported range
+constexpr uint32_t kErrNotInvertible = 2u;  // gcd(x, n^2) != 1 in an inversion
+
+// One Montgomery modulus in R29 form (29-bit limbs, interleaved for its group size).
+struct ModArgs {
+  const uint32_t* m;    // NL words
+  const uint32_t* one;  // R mod m
+  const uint32_t* r2;   // R^2 mod m
+  const uint32_t* r3;   // R^3 mod m (p^2 and q^2 only)
+  uint32_t mprime;      // -m^-1 mod 2^29
+};
+
+// Public-key side, modulus n^2.
+struct PubArgs {
+  ModArgs n2;
+  const uint32_t* n29;    // n as plain 29-bit limbs (NL words)
+  const uint32_t* table;  // fixed-base table of h_s: windows x 2^unit entries x NL words
+  int windows, unit;
+};
+
+// Secret-key side: the two CRT halves mod p^2 and q^2.
+struct DecArgs {
+  ModArgs p2, q2;
+  const uint32_t* sched_p;  // sliding-window schedule of p-1
+  const uint32_t* sched_q;  // ... and of q-1
+  int nsched_p, nsched_q;
+};
+
+// CRT recombination constants, 32-bit words (PW = words of p).
+struct CombArgs {
+  const uint32_t *p, *q;          // PW words
+  const uint32_t *pinv, *qinv;    // p^-1, q^-1 mod 2^(32 PW)
+  const uint32_t *hpR, *hqR;      // hp * 2^(32 PW) mod p, hq * 2^(32 PW) mod q
+  const uint32_t* pinvqR;         // (p^-1 mod q) * 2^(32 PW) mod q
+  const uint32_t *n, *n_half;     // 2 PW words
+  uint32_t pprime, qprime;        // -p^-1, -q^-1 mod 2^32
+};
+
+// One contiguous run of a reduction segment.
+struct Chunk {
+  long start, len;
+};
+
+typedef void (*TableLevelFn)(hipStream_t, const ModArgs&, const uint32_t*, uint32_t*, int, int, int);
+typedef void (*EncI64Fn)(hipStream_t, const PubArgs&, const int64_t*, const uint32_t*, int, long, uint32_t*,
+                         uint32_t*);
+typedef void (*EncBigFn)(hipStream_t, const PubArgs&, const uint32_t*, int, const uint32_t*, int, long, uint32_t*,
+                         uint32_t*);
+typedef void (*RandFn)(hipStream_t, const PubArgs&, const uint32_t*, const uint32_t*, int, long, uint32_t*,
+                       uint32_t*);
+typedef void (*MulModFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
+typedef void (*AddPtFn)(hipStream_t, const PubArgs&, const uint32_t*, const int64_t*, long, uint32_t*);
+typedef void (*PowFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, uint32_t*, u8*, uint32_t*);
+typedef void (*ReduceSumFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const Chunk*, long, uint32_t*);
+typedef void (*ReduceDot0Fn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, const Chunk*, long,
+                             uint32_t*, uint32_t*, uint32_t*);
+typedef void (*ReduceMontFn)(hipStream_t, const ModArgs&, const uint32_t*, const Chunk*, long, uint32_t*);
+typedef void (*SegFinishFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, int, uint32_t*);
+typedef void (*MontTimesZFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
+typedef void (*InvPrefixFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const u8*, long, int, uint32_t*,
+                            uint32_t*);
+typedef void (*InvDownFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const u8*, long, int, uint32_t*,
+                          const uint32_t*, uint32_t*);
+typedef void (*InvSingleFn)(hipStream_t, const ModArgs&, const uint32_t*, uint32_t*, uint32_t*);
+typedef void (*MulConstFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
+typedef void (*DecHalfFn)(hipStream_t, const DecArgs&, const uint32_t*, long, uint32_t*, uint32_t*);
+typedef void (*CombineFn)(hipStream_t, const CombArgs&, const uint32_t*, long, uint32_t*, int, u8*);
+
+// Launchers and shape constants for one key size.
+struct Ops {
+  int n_L, n_K, ct_words;  // group shape mod n^2 and ciphertext words
+  int p_L, p_K, p_words;   // group shape mod p^2 and words of p
+  TableLevelFn table_level;
+  EncI64Fn encrypt_i64;
+  EncBigFn encrypt_big;
+  RandFn randomize;
+  MulModFn mulmod;
+  AddPtFn add_pt_i64;
+  PowFn pow_i64;
+  ReduceSumFn reduce_sum;
+  ReduceDot0Fn reduce_dot0;
+  ReduceMontFn reduce_mont;
+  SegFinishFn seg_finish;
+  MontTimesZFn mont_times_z;
+  InvPrefixFn inv_prefix;
+  InvDownFn inv_down;
+  InvSingleFn inv_single;
+  MulConstFn mul_const;
+  DecHalfFn dec_half;
+  CombineFn combine;
+};
+
+const Ops* ops_for_key_bits(int key_bits);
+
+constexpr int kPowWindowBits = 4;  // fixed window of k_pow_i64 / dot terms
+constexpr int kDecWindowBits = 5;  // sliding window of the CRT exponentiations
+
+}  // namespace heu_hip

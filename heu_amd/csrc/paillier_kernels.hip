@@ -1,0 +1,1172 @@
+// paillier_kernels.hip -- gfx950 kernels of the Paillier batch engine.
+//
+// Every kernel works on residues spread over lane groups (mp_group.hpp).  One
+// group handles one element (or one chunk of a reduction); a block holds
+// kBlock/L groups.  The modulus is staged in LDS once per block.  Each group
+// owns two LDS operand slots: A holds the running value (the multiplier operand
+// of every Montgomery product), B parks a second value; registers only carry
+// the Montgomery working set, which is what lets the wide shapes (K = 27, 36)
+// run without spilling.  Kernels never allocate: scratch buffers come from the
+// host engine.
+#include "kernels.hpp"
+#include "mp_group.hpp"
+
+namespace heu_hip {
+
+constexpr int kBlock = 256;
+constexpr int kPowWin = kPowWindowBits;
+constexpr int kDecWin = kDecWindowBits;
+
+template <int L, int K, int W>
+struct Cfg {
+  static constexpr int NL = L * K;
+  static constexpr int kHalfW = (W + 1) / 2;
+  // slot words: >= NL, and two slots hold a staged W-word number
+  static constexpr int S = (((NL > kHalfW ? NL : kHalfW) + 3) & ~3) + 4;
+  static constexpr int kGroups = kBlock / L;
+  static constexpr int kLdsWords = NL + kGroups * 2 * S;
+  // registers of the Montgomery working set: 2K (accumulators) + K (b) + K (m)
+  static constexpr int kWaves = 4 * K + 40 <= 128 ? 4 : (4 * K + 40 <= 168 ? 3 : 2);
+};
+
+template <int L, int K>
+__device__ __forceinline__ void one_limbs(u32 (&x)[K], u32 lid) {
+#pragma unroll
+  for (int t = 0; t < K; ++t) x[t] = (lid == 0u && t == 0) ? 1u : 0u;
+}
+
+// Group context: the two LDS slots, the modulus and the lane.
+template <int L, int K, int W>
+struct Grp {
+  typedef Cfg<L, K, W> C;
+  u32* A;
+  u32* B;
+  GMod<L, K> md;
+  u32 lid;
+
+  // A <- A * b * R^-1
+  __device__ __forceinline__ void mul(const u32 (&b)[K]) {
+    u64 acc[K];
+    mont_mul<L, K>(acc, A, b, md);
+    u32 x[K];
+    seminorm<L, K>(acc, x);
+    put_lds<L, K>(A, x, lid);
+  }
+  // out <- slot * b * R^-1 (slot unchanged)
+  __device__ __forceinline__ void mul_to(u32 (&out)[K], const u32* slot, const u32 (&b)[K]) {
+    u64 acc[K];
+    mont_mul<L, K>(acc, slot, b, md);
+    seminorm<L, K>(acc, out);
+  }
+  __device__ __forceinline__ void mul_global(const u32* __restrict__ src) {
+    u32 b[K];
+    load_r29<L, K>(b, src, lid);
+    mul(b);
+  }
+  __device__ __forceinline__ void square() {
+    u32 b[K];
+    get_lds<L, K>(b, A, lid);
+    mul(b);
+  }
+  __device__ __forceinline__ void mul_B() {
+    u32 b[K];
+    get_lds<L, K>(b, B, lid);
+    mul(b);
+  }
+  __device__ __forceinline__ void set(u32* slot, const u32 (&x)[K]) { put_lds<L, K>(slot, x, lid); }
+  __device__ __forceinline__ void set_global(u32* slot, const u32* __restrict__ src) {
+    u32 x[K];
+    load_r29<L, K>(x, src, lid);
+    put_lds<L, K>(slot, x, lid);
+  }
+  __device__ __forceinline__ void save(u32* __restrict__ dst, const u32* slot) {
+    u32 x[K];
+    get_lds<L, K>(x, slot, lid);
+    store_r29<L, K>(dst, x, lid);
+  }
+  // A W-word number from global memory into this lane's limbs, staged in B
+  // (or in A..B when one slot is too small).
+  __device__ __forceinline__ void load_words(u32 (&x)[K], const u32* __restrict__ src, int words, int first = 0) {
+    u32* st = words <= C::S ? B : A;
+    stage_words<L>(st, src, words);
+    words_to_limbs<L, K>(st, words, first, x);
+  }
+  // Canonical residue of A written as `words` 32-bit words.
+  __device__ __forceinline__ void store_canonical(u32* __restrict__ dst, int words) {
+    u32 x[K];
+    get_lds<L, K>(x, A, lid);
+    canonicalize<L, K>(x, md);
+    store_words<L, K>(dst, B, x, words);
+  }
+  // A <- A * R^-1 (Montgomery form -> Z-space)
+  __device__ __forceinline__ void from_mont() {
+    u32 b[K];
+    one_limbs<L, K>(b, lid);
+    mul(b);
+  }
+};
+
+// Block prologue: stage the modulus into LDS, then set up this group.
+template <int L, int K, int W>
+__device__ __forceinline__ Grp<L, K, W> grp_setup(u32* lds, const ModArgs& mod, long& e) {
+  typedef Cfg<L, K, W> C;
+  for (int i = threadIdx.x; i < C::NL; i += kBlock) lds[i] = mod.m[i];
+  __syncthreads();
+  Grp<L, K, W> g;
+  g.lid = grp_lane<L>();
+  const int gi = (int)(threadIdx.x / (u32)L);
+  e = (long)blockIdx.x * C::kGroups + gi;
+  g.A = lds + C::NL + gi * 2 * C::S;
+  g.B = g.A + C::S;
+  g.md.m = lds;
+  g.md.mprime = mod.mprime;
+  return g;
+}
+
+#define LDS_DECL(L, K, W) __shared__ u32 lds[Cfg<L, K, W>::kLdsWords]
+
+// ---------------------------------------------------------------------------
+// Fixed-base table of h_s (public_key.cc:36-39 MakeBaseTable):
+// T[w][d] = h_s^(d * 2^(unit*w)) * R mod n^2, R29 interleaved, NL words each.
+// Level 0 writes d = 0 (R mod n^2) and d = 1; level j >= 1 writes d in
+// (h, 2h], h = 2^(j-1), as T[w][d-h] * T[w][h].
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_table_level(ModArgs mod, const u32* __restrict__ bases, u32* table, int windows, int unit,
+                                    int level) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  const long per = 1l << unit;
+  if (level == 0) {
+    if (e >= windows) return;
+    u32 x[K];
+    load_r29<L, K>(x, mod.one, g.lid);
+    store_r29<L, K>(table + (e * per + 0) * NL, x, g.lid);
+    g.load_words(x, bases + e * W, W);
+    g.set(g.A, x);
+    g.mul_global(mod.r2);
+    g.save(table + (e * per + 1) * NL, g.A);
+    return;
+  }
+  const long h = 1l << (level - 1);
+  const long w = e / h, d = h + 1 + e % h;
+  if (w >= windows || d >= per) return;
+  g.set_global(g.A, table + (w * per + (d - h)) * NL);
+  g.mul_global(table + (w * per + h) * NL);
+  g.save(table + (w * per + d) * NL, g.A);
+}
+
+// gm = 1 + n*m mod n^2 for a signed 64-bit m, canonical limbs (encryptor.cc:42-46).
+template <int L, int K>
+__device__ __forceinline__ void gm_from_i64(u32 (&gm)[K], i64 m, const u32* __restrict__ n29, const GMod<L, K>& md,
+                                            u32 lid) {
+  const bool neg = m < 0;
+  const u64 mag = neg ? (u64)0 - (u64)m : (u64)m;
+  const u32 m0 = (u32)mag & kLimbMask, m1 = (u32)(mag >> kLimbBits) & kLimbMask,
+            m2 = (u32)(mag >> (2 * kLimbBits));
+  i64 v[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int j = (int)lid * K + t;
+    u64 p = (u64)n29[j] * m0;
+    if (j >= 1) p += (u64)n29[j - 1] * m1;
+    if (j >= 2) p += (u64)n29[j - 2] * m2;
+    i64 x = neg ? (i64)md.m[t * L + lid] - (i64)p : (i64)p;
+    v[t] = x + (j == 0 ? 1 : 0);
+  }
+  norm_full<L, K>(v, gm);
+}
+
+__device__ __forceinline__ u32 r_digit(const u32* __restrict__ rp, int r_words, int bit, int unit) {
+  const int w0 = bit >> 5, s = bit & 31;
+  u64 lo = w0 < r_words ? rp[w0] : 0u;
+  u64 hi = w0 + 1 < r_words ? rp[w0 + 1] : 0u;
+  return (u32)(((hi << 32) | lo) >> s) & ((1u << unit) - 1u);
+}
+
+// r must fit the table: bits at and above windows*unit are zero.
+__device__ __forceinline__ bool r_fits(const u32* __restrict__ rp, int r_words, int cap_bits) {
+  bool ok = true;
+  for (int w = 0; w < r_words; ++w) {
+    int lo = 32 * w;
+    u32 v = rp[w];
+    if (lo >= cap_bits) ok &= (v == 0u);
+    else if (cap_bits - lo < 32) ok &= (v >> (cap_bits - lo)) == 0u;
+  }
+  return ok;
+}
+
+// A <- h_s^r * R mod n^2 from the table (encryptor.cc:26-31 GetRn).
+template <int L, int K, int W>
+__device__ __forceinline__ void fixed_base_pow(Grp<L, K, W>& g, const PubArgs& P, const u32* __restrict__ rp,
+                                               int r_words) {
+  constexpr int NL = L * K;
+  const long per = 1l << P.unit;
+  g.set_global(g.A, P.table + (long)r_digit(rp, r_words, 0, P.unit) * NL);
+  for (int w = 1; w < P.windows; ++w)
+    g.mul_global(P.table + (w * per + r_digit(rp, r_words, w * P.unit, P.unit)) * NL);
+}
+
+// ---------------------------------------------------------------------------
+// Encryption (encryptor.cc:35-49): c = (1 + n*m) * h_s^r mod n^2, Z-space out.
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_i64(PubArgs P, const i64* __restrict__ m, const u32* __restrict__ r, int r_words,
+                                    long count, u32* __restrict__ out, u32* err) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, P.n2, e);
+  if (e >= count) return;
+  const u32* rp = r + e * r_words;
+  if (!r_fits(rp, r_words, P.windows * P.unit)) {
+    if (g.lid == 0) atomicOr(err, kErrRange);
+    return;
+  }
+  fixed_base_pow(g, P, rp, r_words);
+  u32 gm[K];
+  gm_from_i64<L, K>(gm, m[e], P.n29, g.md, g.lid);
+  g.mul(gm);  // (h_s^r R) * gm * R^-1 = h_s^r * gm
+  g.store_canonical(out + e * W, W);
+}
+
+// Encryption of plaintexts given as two's complement over mw words (|m| <= n/2
+// is checked by the host).  gm = 1 + n*m mod n^2 via two Montgomery products:
+// |m| R (from R^2), then (|m| R) * n * R^-1 = |m| n.
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_big(PubArgs P, const u32* __restrict__ m, int mw, const u32* __restrict__ r,
+                                    int r_words, long count, u32* __restrict__ out, u32* err) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, P.n2, e);
+  if (e >= count) return;
+  const u32* rp = r + e * r_words;
+  if (!r_fits(rp, r_words, P.windows * P.unit)) {
+    if (g.lid == 0) atomicOr(err, kErrRange);
+    return;
+  }
+  // |m| words into B (two's complement negation when the sign bit is set)
+  const u32* mp = m + e * mw;
+  const bool neg = (mp[mw - 1] >> 31) != 0u;
+  wave_lds_sync();
+  if (g.lid == 0) {
+    u64 c = 1;
+    for (int i = 0; i < mw; ++i) {
+      u32 v = mp[i];
+      if (neg) {
+        u64 s = (u64)(u32)~v + c;
+        v = (u32)s;
+        c = s >> 32;
+      }
+      g.B[i] = v;
+    }
+  }
+  wave_lds_sync();
+  u32 x[K];
+  words_to_limbs<L, K>(g.B, mw, 0, x);
+  g.set(g.A, x);
+  g.mul_global(P.n2.r2);  // |m| R
+#pragma unroll
+  for (int t = 0; t < K; ++t) x[t] = P.n29[(int)g.lid * K + t];
+  g.mul(x);  // |m| n (< 2 n^2)
+  get_lds<L, K>(x, g.A, g.lid);
+  canonicalize<L, K>(x, g.md);
+  i64 v[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    const int j = (int)g.lid * K + t;
+    i64 y = neg ? (i64)g.md.m[t * L + g.lid] - (i64)x[t] : (i64)x[t];
+    v[t] = y + (j == 0 ? 1 : 0);
+  }
+  norm_full<L, K>(v, x);  // gm = 1 + n m mod n^2 (canonical; m != 0 when negative)
+  g.set(g.B, x);
+  fixed_base_pow(g, P, rp, r_words);
+  g.mul_B();
+  g.store_canonical(out + e * W, W);
+}
+
+// Randomize (evaluator.cc:25-28) and EncryptZero (encryptor.cc:33):
+// out = c * h_s^r mod n^2 (c = 1 when ct is null).
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_randomize(PubArgs P, const u32* __restrict__ ct, const u32* __restrict__ r, int r_words,
+                                  long count, u32* __restrict__ out, u32* err) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, P.n2, e);
+  if (e >= count) return;
+  const u32* rp = r + e * r_words;
+  if (!r_fits(rp, r_words, P.windows * P.unit)) {
+    if (g.lid == 0) atomicOr(err, kErrRange);
+    return;
+  }
+  u32 c[K];
+  if (ct) g.load_words(c, ct + e * W, W);
+  else one_limbs<L, K>(c, g.lid);
+  g.set(g.B, c);
+  fixed_base_pow(g, P, rp, r_words);
+  g.mul_B();
+  g.store_canonical(out + e * W, W);
+}
+
+// ---------------------------------------------------------------------------
+// ct (+) ct = a*b mod n^2 (evaluator.cc:30-44) and ct (+) pt = c*(1+n*p)
+// (evaluator.cc:46-64).  Z-space in and out: (a*b*R^-1) * R^2 * R^-1.
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mulmod(ModArgs mod, const u32* __restrict__ a, const u32* __restrict__ b, long count,
+                               u32* __restrict__ out) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  u32 x[K];
+  g.load_words(x, a + e * W, W);
+  g.set(g.A, x);
+  g.load_words(x, b + e * W, W);
+  g.mul(x);
+  g.mul_global(mod.r2);
+  g.store_canonical(out + e * W, W);
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_add_pt_i64(PubArgs P, const u32* __restrict__ ct, const i64* __restrict__ pt, long count,
+                                   u32* __restrict__ out) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, P.n2, e);
+  if (e >= count) return;
+  u32 x[K];
+  g.load_words(x, ct + e * W, W);
+  g.set(g.A, x);
+  gm_from_i64<L, K>(x, pt[e], P.n29, g.md, g.lid);
+  g.mul(x);
+  g.mul_global(P.n2.r2);
+  g.store_canonical(out + e * W, W);
+}
+
+// ---------------------------------------------------------------------------
+// Powers with 64-bit exponents, fixed 4-bit windows in Montgomery space.
+// T (16 entries of NL words, global scratch) receives c^d R for d < 16.
+// A holds c R on entry and c^mag R on return.  Leading zero windows are
+// skipped per group.
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+__device__ __forceinline__ void pow_u64(Grp<L, K, W>& g, const ModArgs& mod, u64 mag, u32* __restrict__ T) {
+  constexpr int NL = L * K;
+  u32 x[K];
+  load_r29<L, K>(x, mod.one, g.lid);
+  store_r29<L, K>(T, x, g.lid);
+  get_lds<L, K>(x, g.A, g.lid);
+  store_r29<L, K>(T + NL, x, g.lid);
+  g.set(g.B, x);
+  for (int d = 2; d < (1 << kPowWin); ++d) {
+    g.mul_B();  // A = c^d R
+    g.save(T + d * NL, g.A);
+  }
+  int top = 64 / kPowWin - 1;
+  while (top > 0 && ((mag >> (kPowWin * top)) == 0)) --top;
+  g.set_global(g.A, T + ((u32)(mag >> (kPowWin * top)) & ((1u << kPowWin) - 1u)) * NL);
+  for (int w = top - 1; w >= 0; --w) {
+#pragma unroll 1
+    for (int s = 0; s < kPowWin; ++s) g.square();
+    g.mul_global(T + ((u32)(mag >> (kPowWin * w)) & ((1u << kPowWin) - 1u)) * NL);
+  }
+}
+
+// ct (x) scalar = c^k mod n^2 (evaluator.cc:99-123).  k = 0 gives 1, k = 1
+// returns c unchanged; for k < 0 this writes c^|k| and sets neg[e] so the
+// engine inverts it afterwards.
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_pow_i64(ModArgs mod, const u32* __restrict__ ct, const i64* __restrict__ k, long count,
+                                u32* __restrict__ out, u8* neg, u32* tab) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  const i64 kk = k[e];
+  if (neg && g.lid == 0) neg[e] = kk < 0 ? 1 : 0;
+  if (kk == 0 || kk == 1) {
+    for (int w = (int)g.lid; w < W; w += L) out[e * W + w] = kk == 0 ? (w == 0 ? 1u : 0u) : ct[e * W + w];
+    return;
+  }
+  const u64 mag = kk < 0 ? (u64)0 - (u64)kk : (u64)kk;
+  u32 x[K];
+  g.load_words(x, ct + e * W, W);
+  g.set(g.A, x);
+  g.mul_global(mod.r2);  // c R
+  pow_u64(g, mod, mag, tab + e * (long)(NL << kPowWin));
+  g.from_mont();
+  g.store_canonical(out + e * W, W);
+}
+
+// ---------------------------------------------------------------------------
+// CRT decryption, half h of element e (decryptor.cc:32-53):
+//   u_h = c^(p_h - 1) mod p_h^2, canonical, written as W/2 words.
+// c is brought into Montgomery space mod p^2 as (c_hi * R^3 + c_lo * R^2) R^-1
+// with c = c_hi * 2^(29 NL) + c_lo.  The exponent is shared by all elements;
+// its sliding-window schedule (window 5, 16 odd powers) comes from the host,
+// so control flow is uniform.
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_dec_half(DecArgs D, const u32* __restrict__ ct, long count, u32* __restrict__ halves,
+                                 u32* tab) {
+  constexpr int NL = L * K;
+  constexpr int HW = W / 2;
+  LDS_DECL(L, K, W);
+  const int h = blockIdx.y;
+  const ModArgs& mod = h ? D.q2 : D.p2;
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  const u32* sched = h ? D.sched_q : D.sched_p;
+  const int nsched = h ? D.nsched_q : D.nsched_p;
+  {
+    u32 lo[K], hi[K];
+    stage_words<L>(g.A, ct + e * W, W);  // spans A..B
+    words_to_limbs<L, K>(g.A, W, 0, lo);
+    words_to_limbs<L, K>(g.A, W, NL, hi);
+    g.set(g.A, hi);
+    g.set(g.B, lo);
+  }
+  g.mul_global(mod.r3);  // A = c_hi R^2
+  {
+    u32 b[K], x[K];
+    load_r29<L, K>(b, mod.r2, g.lid);
+    g.mul_to(x, g.B, b);  // c_lo R
+    get_lds<L, K>(b, g.A, g.lid);
+    u64 s[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) s[t] = (u64)x[t] + b[t];
+    seminorm<L, K>(s, x);  // c R mod p^2 (lazy, < 3 p^2)
+    g.set(g.A, x);
+  }
+  u32* T = tab + (h * count + e) * (long)(NL << (kDecWin - 1));
+  g.save(T, g.A);
+  {
+    u32 b[K], x2[K];
+    get_lds<L, K>(b, g.A, g.lid);
+    g.mul_to(x2, g.A, b);  // x^2
+    g.set(g.B, x2);
+  }
+  for (int i = 1; i < (1 << (kDecWin - 1)); ++i) {
+    g.mul_B();
+    g.save(T + i * NL, g.A);
+  }
+  // schedule: op = (table index + 1) << 16 | squarings before the multiply
+  u32 op = sched[0];
+  g.set_global(g.A, T + ((op >> 16) - 1) * NL);
+  for (int i = 1; i < nsched; ++i) {
+    op = sched[i];
+    const u32 sq = op & 0xffffu, idx = op >> 16;
+    for (u32 s = 0; s < sq; ++s) g.square();
+    if (idx) g.mul_global(T + (idx - 1) * NL);
+  }
+  g.from_mont();
+  g.store_canonical(halves + (h * count + e) * HW, HW);
+}
+
+// ---------------------------------------------------------------------------
+// CRT recombination, one thread per element, 32-bit words (decryptor.cc:38-52):
+//   mp = L_p(u_p) * hp mod p,  mq = L_q(u_q) * hq mod q,
+//   m = mp + p * ((mq - mp) * p^-1 mod q),  m -= n if m > n/2.
+// L_p(u) = (u - 1) / p is exact: (u - 1) * p^-1 mod 2^(32 PW).
+// ---------------------------------------------------------------------------
+template <int PW>
+__device__ __forceinline__ void mont32(u32 (&r)[PW], const u32 (&a)[PW], const u32* __restrict__ b,
+                                       const u32* __restrict__ m, u32 mp) {
+  u32 t[PW + 2];
+#pragma unroll
+  for (int i = 0; i < PW + 2; ++i) t[i] = 0;
+  for (int i = 0; i < PW; ++i) {
+    u64 c = 0;
+    const u32 ai = a[i];
+#pragma unroll
+    for (int j = 0; j < PW; ++j) {
+      u64 v = (u64)ai * b[j] + t[j] + c;
+      t[j] = (u32)v;
+      c = v >> 32;
+    }
+    u64 v = (u64)t[PW] + c;
+    t[PW] = (u32)v;
+    t[PW + 1] = (u32)(v >> 32);
+    const u32 q = t[0] * mp;
+    v = (u64)q * m[0] + t[0];
+    c = v >> 32;
+#pragma unroll
+    for (int j = 1; j < PW; ++j) {
+      v = (u64)q * m[j] + t[j] + c;
+      t[j - 1] = (u32)v;
+      c = v >> 32;
+    }
+    v = (u64)t[PW] + c;
+    t[PW - 1] = (u32)v;
+    t[PW] = t[PW + 1] + (u32)(v >> 32);
+  }
+  // t < 2m: subtract m once if t >= m
+  u32 d[PW];
+  i64 br = 0;
+#pragma unroll
+  for (int j = 0; j < PW; ++j) {
+    i64 v = (i64)t[j] - m[j] + br;
+    d[j] = (u32)v;
+    br = v >> 32;
+  }
+  const bool ge = (i64)t[PW] + br >= 0;
+#pragma unroll
+  for (int j = 0; j < PW; ++j) r[j] = ge ? d[j] : t[j];
+}
+
+template <int PW>
+__device__ __forceinline__ void l_times_h(u32 (&out)[PW], const u32* __restrict__ u, const u32* __restrict__ pinv,
+                                          const u32* __restrict__ hR, const u32* __restrict__ p, u32 pp) {
+  u32 um1[PW];
+  i64 br = -1;
+#pragma unroll
+  for (int j = 0; j < PW; ++j) {
+    i64 v = (i64)u[j] + br;
+    um1[j] = (u32)v;
+    br = v >> 32;
+  }
+  u32 l[PW];
+#pragma unroll
+  for (int j = 0; j < PW; ++j) l[j] = 0;
+  for (int i = 0; i < PW; ++i) {
+    u64 c = 0;
+    const u32 ai = um1[i];
+    for (int j = 0; i + j < PW; ++j) {
+      u64 v = (u64)ai * pinv[j] + l[i + j] + c;
+      l[i + j] = (u32)v;
+      c = v >> 32;
+    }
+  }
+  mont32<PW>(out, l, hR, p, pp);  // L * h mod p
+}
+
+template <int PW>
+__global__ __launch_bounds__(kBlock) void k_dec_combine(CombArgs C, const u32* __restrict__ halves, long count,
+                                                        u32* __restrict__ out, int out_words, u8* overflow) {
+  const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= count) return;
+  constexpr int HW = 2 * PW;
+  u32 mp[PW], mq[PW];
+  l_times_h<PW>(mp, halves + e * HW, C.pinv, C.hpR, C.p, C.pprime);
+  l_times_h<PW>(mq, halves + (count + e) * HW, C.qinv, C.hqR, C.q, C.qprime);
+  // d = mq + 2q - mp in [1, 3q): reduce below q
+  u32 d[PW];
+  i64 c = 0;
+#pragma unroll
+  for (int j = 0; j < PW; ++j) {
+    i64 v = (i64)mq[j] + 2 * (i64)C.q[j] - (i64)mp[j] + c;
+    d[j] = (u32)v;
+    c = v >> 32;
+  }
+  i64 dtop = c;
+  for (int rep = 0; rep < 2; ++rep) {
+    u32 s[PW];
+    i64 b = 0;
+#pragma unroll
+    for (int j = 0; j < PW; ++j) {
+      i64 v = (i64)d[j] - (i64)C.q[j] + b;
+      s[j] = (u32)v;
+      b = v >> 32;
+    }
+    if (dtop + b >= 0) {
+#pragma unroll
+      for (int j = 0; j < PW; ++j) d[j] = s[j];
+      dtop += b;
+    }
+  }
+  u32 t[PW];
+  mont32<PW>(t, d, C.pinvqR, C.q, C.qprime);  // (mq - mp) p^-1 mod q
+  u32 m[2 * PW];
+#pragma unroll
+  for (int j = 0; j < 2 * PW; ++j) m[j] = j < PW ? mp[j] : 0u;
+  for (int i = 0; i < PW; ++i) {
+    u64 cc = 0;
+    const u32 ti = t[i];
+    for (int j = 0; j < PW; ++j) {
+      u64 v = (u64)ti * C.p[j] + m[i + j] + cc;
+      m[i + j] = (u32)v;
+      cc = v >> 32;
+    }
+    for (int j = i + PW; j < 2 * PW && cc; ++j) {
+      u64 v = (u64)m[j] + cc;
+      m[j] = (u32)v;
+      cc = v >> 32;
+    }
+  }
+  // sign fold: m > n/2 -> m - n (two's complement)
+  i64 b = 0;
+#pragma unroll
+  for (int j = 0; j < 2 * PW; ++j) {
+    i64 v = (i64)C.n_half[j] - (i64)m[j] + b;
+    b = v >> 32;
+  }
+  const bool fold = b < 0;
+  if (fold) {
+    i64 bb = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * PW; ++j) {
+      i64 v = (i64)m[j] - (i64)C.n[j] + bb;
+      m[j] = (u32)v;
+      bb = v >> 32;
+    }
+  }
+  const u32 ext = fold ? 0xffffffffu : 0u;
+  bool fits = true;
+  for (int j = out_words; j < 2 * PW; ++j) fits &= (m[j] == ext);
+  if (out_words <= 2 * PW) fits &= ((m[out_words - 1] >> 31) == (ext & 1u));
+  for (int j = 0; j < out_words; ++j) out[e * out_words + j] = j < 2 * PW ? m[j] : ext;
+  if (overflow) overflow[e] = fits ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Segment reductions (numpy/evaluator.cc:355-373 Sum, :209-292 MatMul dot).
+// A chunk is a contiguous run of one segment; one group reduces one chunk.
+//  - sum: Montgomery products of Z-space inputs; a product of N terms through
+//    N-1 multiplications carries R^-(N-1), removed by k_seg_finish.
+//  - dot (level 0): term = (c R)^|w| in Montgomery space, multiplied into a
+//    positive or negative accumulator by the sign of w (w = 0 contributes 1).
+// Partials are R29 interleaved.
+// ---------------------------------------------------------------------------
+template <int L, int K, int W, bool ZIN>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_sum(ModArgs mod, const u32* __restrict__ in, const Chunk* __restrict__ chunks,
+                                   long nchunks, u32* __restrict__ part) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= nchunks) return;
+  const Chunk ch = chunks[e];
+  u32 x[K];
+  if (ch.len == 0) {
+    one_limbs<L, K>(x, g.lid);
+    store_r29<L, K>(part + e * NL, x, g.lid);
+    return;
+  }
+  if constexpr (ZIN) g.load_words(x, in + ch.start * W, W);
+  else load_r29<L, K>(x, in + ch.start * NL, g.lid);
+  g.set(g.A, x);
+  for (long i = ch.start + 1; i < ch.start + ch.len; ++i) {
+    if constexpr (ZIN) g.load_words(x, in + i * W, W);
+    else load_r29<L, K>(x, in + i * NL, g.lid);
+    g.mul(x);
+  }
+  g.save(part + e * NL, g.A);
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_dot0(ModArgs mod, const u32* __restrict__ in, const i64* __restrict__ wts,
+                                    const Chunk* __restrict__ chunks, long nchunks, u32* __restrict__ part_pos,
+                                    u32* __restrict__ part_neg, u32* tab) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= nchunks) return;
+  const Chunk ch = chunks[e];
+  u32* pp = part_pos + e * NL;
+  u32* pn = part_neg + e * NL;
+  {
+    u32 one[K];
+    load_r29<L, K>(one, mod.one, g.lid);
+    store_r29<L, K>(pp, one, g.lid);
+    store_r29<L, K>(pn, one, g.lid);
+  }
+  u32* T = tab + e * (long)(NL << kPowWin);
+  for (long i = ch.start; i < ch.start + ch.len; ++i) {
+    const i64 w = wts[i];
+    if (w == 0) continue;
+    const u64 mag = w < 0 ? (u64)0 - (u64)w : (u64)w;
+    u32 x[K];
+    g.load_words(x, in + i * W, W);
+    g.set(g.A, x);
+    g.mul_global(mod.r2);  // c R
+    if (mag != 1) pow_u64(g, mod, mag, T);
+    // accumulator <- accumulator * term; accumulators live in global memory
+    u32* acc = w > 0 ? pp : pn;
+    get_lds<L, K>(x, g.A, g.lid);
+    g.set_global(g.A, acc);
+    g.mul(x);
+    g.save(acc, g.A);
+  }
+}
+
+// Upper levels of a dot: products of Montgomery-form partials (stay in M-space).
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_mont(ModArgs mod, const u32* __restrict__ in, const Chunk* __restrict__ chunks,
+                                    long nchunks, u32* __restrict__ part) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= nchunks) return;
+  const Chunk ch = chunks[e];
+  g.set_global(g.A, mod.one);
+  for (long i = ch.start; i < ch.start + ch.len; ++i) g.mul_global(in + i * NL);
+  g.save(part + e * NL, g.A);
+}
+
+// Finish a segment sum: the total carries R^-(N-1); multiply by the
+// Montgomery form of R^(N-1), i.e. R^N, built as a Montgomery power of R^2
+// (the Montgomery form of R).  With mont_in set, the partial is a
+// Montgomery-form value and is only mapped back to Z-space.
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_seg_finish(ModArgs mod, const u32* __restrict__ part, const i64* __restrict__ n_terms,
+                                   long nseg, int mont_in, u32* __restrict__ out) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= nseg) return;
+  if (mont_in) {
+    g.set_global(g.A, part + e * NL);
+    g.from_mont();
+  } else {
+    const i64 nt = n_terms[e];
+    const u64 ex = nt > 1 ? (u64)(nt - 1) : 0u;
+    g.set_global(g.A, mod.one);
+    int top = 63;
+    while (top > 0 && ((ex >> top) & 1u) == 0) --top;
+    for (int bit = top; bit >= 0; --bit) {
+      g.square();
+      if ((ex >> bit) & 1u) g.mul_global(mod.r2);
+    }
+    u32 x[K];
+    get_lds<L, K>(x, g.A, g.lid);
+    g.set_global(g.A, part + e * NL);
+    g.mul(x);  // total * R^(ex+1) * R^-1
+  }
+  g.store_canonical(out + e * W, W);
+}
+
+// out = x * y mod m where x is Montgomery form (R29) and y is Z-space words:
+// x R * y * R^-1 = x y.  Finishes a dot as P * N^-1.
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mont_times_z(ModArgs mod, const u32* __restrict__ xm, const u32* __restrict__ yz, long count,
+                                     u32* __restrict__ out) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  u32 y[K];
+  g.load_words(y, yz + e * W, W);
+  g.set_global(g.A, xm + e * NL);
+  g.mul(y);
+  g.store_canonical(out + e * W, W);
+}
+
+// ---------------------------------------------------------------------------
+// Batch inversion (Montgomery's trick) for Negate / Sub / negative scalars
+// (evaluator.cc:66-97).  Level-0 inputs are Z-space words selected by `sel`
+// (unselected elements count as 1 and are copied through); upper levels work
+// on Montgomery-form R29 values.
+//   prefix: per chunk, P_j = X_0 ... X_j (Montgomery form), chunk total out;
+//   down:   given the inverse of the chunk total, y_j = inv * P_(j-1),
+//           inv = inv * X_j, y_0 = inv.
+// ---------------------------------------------------------------------------
+// x <- X_i = x_i R (Montgomery form of element i), via slot B.
+template <int L, int K, int W, bool ZIN>
+__device__ __forceinline__ bool inv_x(u32 (&x)[K], Grp<L, K, W>& g, const ModArgs& mod, const u32* in,
+                                      const u8* __restrict__ sel, long i) {
+  constexpr int NL = L * K;
+  if constexpr (ZIN) {
+    if (sel && !sel[i]) return false;
+    g.load_words(x, in + i * W, W);
+    g.set(g.B, x);
+    u32 b[K];
+    load_r29<L, K>(b, mod.r2, g.lid);
+    g.mul_to(x, g.B, b);
+  } else {
+    load_r29<L, K>(x, in + i * NL, g.lid);
+  }
+  return true;
+}
+
+template <bool ZIN>
+__device__ __forceinline__ bool chunk_selected(const u8* __restrict__ sel, long first, long last) {
+  if (!ZIN || !sel) return true;
+  bool any = false;
+  for (long i = first; i < last; ++i) any |= sel[i] != 0;
+  return any;
+}
+
+template <int L, int K, int W, bool ZIN>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_prefix(ModArgs mod, const u32* __restrict__ in, const u8* __restrict__ sel, long count,
+                                   int chunk, u32* __restrict__ prefix, u32* __restrict__ totals) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  const long first = e * chunk;
+  if (first >= count) return;
+  const long last = first + chunk < count ? first + chunk : count;
+  g.set_global(g.A, mod.one);
+  if (chunk_selected<ZIN>(sel, first, last)) {
+    for (long i = first; i < last; ++i) {
+      u32 x[K];
+      if (inv_x<L, K, W, ZIN>(x, g, mod, in, sel, i)) g.mul(x);  // unselected: X = 1
+      g.save(prefix + i * NL, g.A);
+    }
+  }
+  g.save(totals + e * NL, g.A);
+}
+
+// Down pass.  `out` may alias `in`: element i is read (to rebuild X_i) before
+// out[i] is written.  y_i is parked in the spent prefix slot P_(i-1).
+template <int L, int K, int W, bool ZIN>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(ModArgs mod, const u32* in,
+                                                                           const u8* __restrict__ sel, long count,
+                                                                           int chunk, u32* prefix,
+                                                                           const u32* __restrict__ tot_inv, u32* out) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  const long first = e * chunk;
+  if (first >= count) return;
+  const long last = first + chunk < count ? first + chunk : count;
+  if (!chunk_selected<ZIN>(sel, first, last)) {
+    if constexpr (ZIN) {
+      if (out != in)
+        for (long i = first; i < last; ++i)
+          for (int w = (int)g.lid; w < W; w += L) out[i * W + w] = in[i * W + w];
+    }
+    return;
+  }
+  g.set_global(g.A, tot_inv + e * NL);  // A = inv (Montgomery form)
+  for (long i = last - 1; i >= first; --i) {
+    if constexpr (ZIN) {
+      if (sel && !sel[i]) {
+        if (out != in)
+          for (int w = (int)g.lid; w < W; w += L) out[i * W + w] = in[i * W + w];
+        continue;
+      }
+    }
+    u32* ypark = nullptr;
+    if (i > first) {
+      u32 x[K];
+      inv_x<L, K, W, ZIN>(x, g, mod, in, sel, i);  // X_i, read before out[i] is written
+      g.set(g.B, x);
+      ypark = prefix + (i - 1) * NL;
+      u32 b[K];
+      load_r29<L, K>(b, ypark, g.lid);
+      g.mul_to(x, g.A, b);  // y_i = X_i^-1 R = inv * P_(i-1)
+      store_r29<L, K>(ypark, x, g.lid);
+      g.mul_B();  // inv <- inv * X_i
+    }
+    u32 y[K];
+    if (ypark) load_r29<L, K>(y, ypark, g.lid);
+    else get_lds<L, K>(y, g.A, g.lid);
+    if constexpr (ZIN) {
+      // to Z-space and out; B is the operand and then the staging area
+      g.set(g.B, y);
+      u32 one[K];
+      one_limbs<L, K>(one, g.lid);
+      g.mul_to(y, g.B, one);
+      canonicalize<L, K>(y, g.md);
+      store_words<L, K>(out + i * W, g.B, y, W);
+    } else {
+      store_r29<L, K>(out + i * NL, y, g.lid);
+    }
+  }
+}
+
+// Inverse of one Montgomery-form value (R29), in place: to Z-space, binary
+// extended Euclid on 32-bit words by one thread, back to Montgomery form.
+// Sets kErrNotInvertible if gcd(x, m) != 1.
+template <int L, int K, int W>
+__global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __restrict__ m_words, u32* xio, u32* err) {
+  constexpr int NL = L * K;
+  __shared__ u32 lds[NL + 2 * Cfg<L, K, W>::S];
+  __shared__ u32 u[W + 1], v[W + 1], x1[W + 1], x2[W + 1], mm[W + 1];
+  __shared__ u32 fail;
+  for (int i = threadIdx.x; i < NL; i += 64) lds[i] = mod.m[i];
+  __syncthreads();
+  Grp<L, K, W> g;
+  g.lid = grp_lane<L>();
+  g.A = lds + NL;
+  g.B = g.A + Cfg<L, K, W>::S;
+  g.md.m = lds;
+  g.md.mprime = mod.mprime;
+  const bool in_grp = threadIdx.x < (u32)L;
+  if (in_grp) {
+    g.set_global(g.A, xio);
+    g.from_mont();
+    u32 x[K];
+    get_lds<L, K>(x, g.A, g.lid);
+    canonicalize<L, K>(x, g.md);
+    g.set(g.A, x);
+    for (int w = (int)g.lid; w < W; w += L) u[w] = limbs_to_word<L, K>(g.A, w);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < W; ++i) {
+      v[i] = m_words[i];
+      mm[i] = m_words[i];
+      x1[i] = i == 0 ? 1u : 0u;
+      x2[i] = 0u;
+    }
+    auto is_one = [&](const u32* a) {
+      if (a[0] != 1u) return false;
+      for (int i = 1; i < W; ++i)
+        if (a[i]) return false;
+      return true;
+    };
+    auto is_zero = [&](const u32* a) {
+      for (int i = 0; i < W; ++i)
+        if (a[i]) return false;
+      return true;
+    };
+    auto halve = [&](u32* a) {
+      for (int i = 0; i < W; ++i) a[i] = (a[i] >> 1) | (i + 1 < W ? (a[i + 1] << 31) : 0u);
+    };
+    auto halve_mod = [&](u32* a) {  // a/2 mod m (a < m, m odd)
+      u64 c = 0;
+      if (a[0] & 1u) {
+        for (int i = 0; i < W; ++i) {
+          u64 s = (u64)a[i] + mm[i] + c;
+          a[i] = (u32)s;
+          c = s >> 32;
+        }
+      }
+      for (int i = 0; i < W; ++i) a[i] = (a[i] >> 1) | (i + 1 < W ? (a[i + 1] << 31) : ((u32)c << 31));
+    };
+    auto geq = [&](const u32* a, const u32* bb) {
+      for (int i = W - 1; i >= 0; --i)
+        if (a[i] != bb[i]) return a[i] > bb[i];
+      return true;
+    };
+    auto sub = [&](u32* a, const u32* bb) {
+      i64 br = 0;
+      for (int i = 0; i < W; ++i) {
+        i64 s = (i64)a[i] - bb[i] + br;
+        a[i] = (u32)s;
+        br = s >> 32;
+      }
+    };
+    auto sub_mod = [&](u32* a, const u32* bb) {  // a - b mod m for a, b < m
+      i64 br = 0;
+      for (int i = 0; i < W; ++i) {
+        i64 s = (i64)a[i] - bb[i] + br;
+        a[i] = (u32)s;
+        br = s >> 32;
+      }
+      if (br < 0) {
+        u64 c = 0;
+        for (int i = 0; i < W; ++i) {
+          u64 s = (u64)a[i] + mm[i] + c;
+          a[i] = (u32)s;
+          c = s >> 32;
+        }
+      }
+    };
+    u32 f = is_zero(u) ? 1u : 0u;
+    long guard = 0;
+    const long kMaxSteps = 4l * 32 * W + 64;
+    while (!f && !is_one(u) && !is_one(v)) {
+      while (!(u[0] & 1u)) {
+        halve(u);
+        halve_mod(x1);
+      }
+      while (!(v[0] & 1u)) {
+        halve(v);
+        halve_mod(x2);
+      }
+      if (geq(u, v)) {
+        sub(u, v);
+        sub_mod(x1, x2);
+      } else {
+        sub(v, u);
+        sub_mod(x2, x1);
+      }
+      if (is_zero(u) || is_zero(v) || ++guard > kMaxSteps) f = 1u;
+    }
+    if (!f && !is_one(u))
+      for (int i = 0; i < W; ++i) x1[i] = x2[i];
+    fail = f;
+  }
+  __syncthreads();
+  if (fail) {
+    if (threadIdx.x == 0) atomicOr(err, kErrNotInvertible);
+    return;
+  }
+  if (in_grp) {
+    u32 x[K];
+    words_to_limbs<L, K>(x1, W, 0, x);
+    g.set(g.A, x);
+    g.mul_global(mod.r2);
+    g.save(xio, g.A);
+  }
+}
+
+// Montgomery-form export (encryptor.cc:46 MapIntoMSpace) for R_x = 2^rbits:
+// out = c * cz mod m with cz = R_x mod m (or R_x^-1 mod m for the import),
+// computed as mont(mont(c, cz), R^2).
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mul_const(ModArgs mod, const u32* __restrict__ in, const u32* __restrict__ cz, long count,
+                                  u32* __restrict__ out) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  u32 x[K];
+  g.load_words(x, in + e * W, W);
+  g.set(g.A, x);
+  g.load_words(x, cz, W);
+  g.mul(x);
+  g.mul_global(mod.r2);
+  g.store_canonical(out + e * W, W);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers, one instantiation per supported shape
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+struct Shape {
+  static constexpr int kL = L, kK = K, kW = W, kNL = L * K;
+};
+
+static inline unsigned blocks_for(long groups, int L) {
+  const long per = kBlock / L;
+  return (unsigned)((groups + per - 1) / per);
+}
+
+#define HEU_LAUNCH(kern, groups, L, stream, ...)                                                   \
+  do {                                                                                             \
+    if ((groups) > 0) hipLaunchKernelGGL(kern, dim3(blocks_for((groups), (L))), dim3(kBlock), 0, \
+                                         (stream), __VA_ARGS__);                                   \
+  } while (0)
+
+template <class S>
+struct Launch {
+  static constexpr int L = S::kL, K = S::kK, W = S::kW;
+  static void table_level(hipStream_t st, const ModArgs& mod, const u32* bases, u32* table, int windows, int unit,
+                          int level) {
+    long groups = level == 0 ? windows : (long)windows * (1l << (level - 1));
+    HEU_LAUNCH((k_table_level<L, K, W>), groups, L, st, mod, bases, table, windows, unit, level);
+  }
+  static void encrypt_i64(hipStream_t st, const PubArgs& P, const i64* m, const u32* r, int rw, long n, u32* out,
+                          u32* err) {
+    HEU_LAUNCH((k_encrypt_i64<L, K, W>), n, L, st, P, m, r, rw, n, out, err);
+  }
+  static void encrypt_big(hipStream_t st, const PubArgs& P, const u32* m, int mw, const u32* r, int rw, long n,
+                          u32* out, u32* err) {
+    HEU_LAUNCH((k_encrypt_big<L, K, W>), n, L, st, P, m, mw, r, rw, n, out, err);
+  }
+  static void randomize(hipStream_t st, const PubArgs& P, const u32* ct, const u32* r, int rw, long n, u32* out,
+                        u32* err) {
+    HEU_LAUNCH((k_randomize<L, K, W>), n, L, st, P, ct, r, rw, n, out, err);
+  }
+  static void mulmod(hipStream_t st, const ModArgs& mod, const u32* a, const u32* b, long n, u32* out) {
+    HEU_LAUNCH((k_mulmod<L, K, W>), n, L, st, mod, a, b, n, out);
+  }
+  static void add_pt_i64(hipStream_t st, const PubArgs& P, const u32* ct, const i64* pt, long n, u32* out) {
+    HEU_LAUNCH((k_add_pt_i64<L, K, W>), n, L, st, P, ct, pt, n, out);
+  }
+  static void pow_i64(hipStream_t st, const ModArgs& mod, const u32* ct, const i64* k, long n, u32* out, u8* neg,
+                      u32* tab) {
+    HEU_LAUNCH((k_pow_i64<L, K, W>), n, L, st, mod, ct, k, n, out, neg, tab);
+  }
+  static void reduce_sum(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const Chunk* chunks, long nch,
+                         u32* part) {
+    if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part);
+    else HEU_LAUNCH((k_reduce_sum<L, K, W, false>), nch, L, st, mod, in, chunks, nch, part);
+  }
+  static void reduce_dot0(hipStream_t st, const ModArgs& mod, const u32* in, const i64* w, const Chunk* chunks,
+                          long nch, u32* pp, u32* pn, u32* tab) {
+    HEU_LAUNCH((k_reduce_dot0<L, K, W>), nch, L, st, mod, in, w, chunks, nch, pp, pn, tab);
+  }
+  static void reduce_mont(hipStream_t st, const ModArgs& mod, const u32* in, const Chunk* chunks, long nch,
+                          u32* part) {
+    HEU_LAUNCH((k_reduce_mont<L, K, W>), nch, L, st, mod, in, chunks, nch, part);
+  }
+  static void seg_finish(hipStream_t st, const ModArgs& mod, const u32* part, const i64* nt, long nseg, int mont_in,
+                         u32* out) {
+    HEU_LAUNCH((k_seg_finish<L, K, W>), nseg, L, st, mod, part, nt, nseg, mont_in, out);
+  }
+  static void mont_times_z(hipStream_t st, const ModArgs& mod, const u32* xm, const u32* yz, long n, u32* out) {
+    HEU_LAUNCH((k_mont_times_z<L, K, W>), n, L, st, mod, xm, yz, n, out);
+  }
+  static void inv_prefix(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const u8* sel, long n, int chunk,
+                         u32* prefix, u32* totals) {
+    long groups = (n + chunk - 1) / chunk;
+    if (zin) HEU_LAUNCH((k_inv_prefix<L, K, W, true>), groups, L, st, mod, in, sel, n, chunk, prefix, totals);
+    else HEU_LAUNCH((k_inv_prefix<L, K, W, false>), groups, L, st, mod, in, sel, n, chunk, prefix, totals);
+  }
+  static void inv_down(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const u8* sel, long n, int chunk,
+                       u32* prefix, const u32* tot_inv, u32* out) {
+    long groups = (n + chunk - 1) / chunk;
+    if (zin) HEU_LAUNCH((k_inv_down<L, K, W, true>), groups, L, st, mod, in, sel, n, chunk, prefix, tot_inv, out);
+    else HEU_LAUNCH((k_inv_down<L, K, W, false>), groups, L, st, mod, in, sel, n, chunk, prefix, tot_inv, out);
+  }
+  static void inv_single(hipStream_t st, const ModArgs& mod, const u32* m_words, u32* xio, u32* err) {
+    hipLaunchKernelGGL((k_inv_single<L, K, W>), dim3(1), dim3(64), 0, st, mod, m_words, xio, err);
+  }
+  static void mul_const(hipStream_t st, const ModArgs& mod, const u32* in, const u32* cz, long n, u32* out) {
+    HEU_LAUNCH((k_mul_const<L, K, W>), n, L, st, mod, in, cz, n, out);
+  }
+  static void dec_half(hipStream_t st, const DecArgs& D, const u32* ct, long n, u32* halves, u32* tab) {
+    if (n > 0)
+      hipLaunchKernelGGL((k_dec_half<L, K, W>), dim3(blocks_for(n, L), 2), dim3(kBlock), 0, st, D, ct, n, halves, tab);
+  }
+};
+
+template <int PW>
+static void launch_combine(hipStream_t st, const CombArgs& C, const u32* halves, long n, u32* out, int ow, u8* ovf) {
+  if (n > 0)
+    hipLaunchKernelGGL((k_dec_combine<PW>), dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, C,
+                       halves, n, out, ow, ovf);
+}
+
+// Shapes (29-bit limbs, 29*NL >= bits(m) + 2):
+//   2048-bit keys: n^2 over 4 lanes x 36 limbs, p^2 over 2 lanes x 36 limbs;
+//   3072-bit keys: n^2 over 8 lanes x 27 limbs, p^2 over 4 lanes x 27 limbs.
+// W = words of the ciphertext (decryption halves hold W/2).
+typedef Shape<4, 36, 128> ShapeN2048;
+typedef Shape<8, 27, 192> ShapeN3072;
+typedef Shape<2, 36, 128> ShapeP2048;
+typedef Shape<4, 27, 192> ShapeP3072;
+
+template <class SN, class SP, int PW>
+static const Ops make_ops() {
+  Ops o;
+  o.n_L = SN::kL;
+  o.n_K = SN::kK;
+  o.ct_words = SN::kW;
+  o.p_L = SP::kL;
+  o.p_K = SP::kK;
+  o.p_words = PW;
+  o.table_level = Launch<SN>::table_level;
+  o.encrypt_i64 = Launch<SN>::encrypt_i64;
+  o.encrypt_big = Launch<SN>::encrypt_big;
+  o.randomize = Launch<SN>::randomize;
+  o.mulmod = Launch<SN>::mulmod;
+  o.add_pt_i64 = Launch<SN>::add_pt_i64;
+  o.pow_i64 = Launch<SN>::pow_i64;
+  o.reduce_sum = Launch<SN>::reduce_sum;
+  o.reduce_dot0 = Launch<SN>::reduce_dot0;
+  o.reduce_mont = Launch<SN>::reduce_mont;
+  o.seg_finish = Launch<SN>::seg_finish;
+  o.mont_times_z = Launch<SN>::mont_times_z;
+  o.inv_prefix = Launch<SN>::inv_prefix;
+  o.inv_down = Launch<SN>::inv_down;
+  o.inv_single = Launch<SN>::inv_single;
+  o.mul_const = Launch<SN>::mul_const;
+  o.dec_half = Launch<SP>::dec_half;
+  o.combine = launch_combine<PW>;
+  return o;
+}
+
+const Ops* ops_for_key_bits(int key_bits) {
+  static const Ops k2048 = make_ops<ShapeN2048, ShapeP2048, 32>();
+  static const Ops k3072 = make_ops<ShapeN3072, ShapeP3072, 48>();
+  if (key_bits == 2048) return &k2048;
+  if (key_bits == 3072) return &k3072;
+  return nullptr;
+}
+
+}  // namespace heu_hip

@@ -1,0 +1,329 @@
+"""Python front end of the engine: contexts, resident keys and batch ops on
+device tensors.
+
+Device memory and the stream come from PyTorch (plumbing only); every
+operation runs in libheu_paillier.so.  Ciphertexts are int32 tensors of shape
+(N, ct_words) holding little-endian uint32 limbs of the Z-space value
+c = (1 + n*m) * h_s^r mod n^2 (paillier_zahlen/encryptor.cc:42-49).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+from ._native import check, lib
+
+# ---------------------------------------------------------------------------
+# limbs <-> Python ints
+# ---------------------------------------------------------------------------
+
+
+def int_to_words(v: int, words: int) -> np.ndarray:
+    return np.frombuffer(int(v).to_bytes(words * 4, "little"), dtype=np.uint32).copy()
+
+
+def words_to_int(a) -> int:
+    return int.from_bytes(np.ascontiguousarray(a, dtype=np.uint32).tobytes(), "little")
+
+
+def twos_to_int(a) -> int:
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    v = words_to_int(a)
+    bits = 32 * a.shape[-1]
+    return v - (1 << bits) if v >> (bits - 1) else v
+
+
+def ints_to_words(values, words: int, signed: bool = False) -> np.ndarray:
+    mod = 1 << (32 * words)
+    out = np.empty((len(values), words), dtype=np.uint32)
+    for i, v in enumerate(values):
+        v = int(v)
+        if signed:
+            v %= mod
+        elif v < 0 or v >= mod:
+            raise ValueError("value does not fit")
+        out[i] = np.frombuffer(v.to_bytes(words * 4, "little"), dtype=np.uint32)
+    return out
+
+
+def words_to_ints(rows, signed: bool = False):
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    f = twos_to_int if signed else words_to_int
+    return [f(r) for r in rows]
+
+
+# ---------------------------------------------------------------------------
+# Seeded counter PRNG (same definition as the engine's key generator):
+#   word(seed, stream, i) = mix(mix(seed*G + mix(stream + C)) + (i+1)*G)
+# ---------------------------------------------------------------------------
+_G = np.uint64(0x9E3779B97F4A7C15)
+_C = np.uint64(0x632BE59BD9B4E019)
+
+
+def _mix64(z):
+    z = np.asarray(z, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def prng_words(seed: int, stream: int, count: int, offset: int = 0) -> np.ndarray:
+    """`count` 64-bit words of stream `stream` under `seed`, from index `offset`."""
+    with np.errstate(over="ignore"):
+        key = _mix64(np.uint64(seed) * _G + _mix64(np.uint64(stream) + _C))
+        idx = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+        return _mix64(key + idx * _G)
+
+
+def seeded_r(seed: int, stream: int, count: int, r_bits: int = 256, offset: int = 0) -> np.ndarray:
+    """Randomness r uniform in [1, 2^r_bits) as (count, ceil(r_bits/32)) uint32 limbs
+    (r = 0 is replaced by 1)."""
+    w64 = (r_bits + 63) // 64
+    raw = prng_words(seed, stream, count * w64, offset * w64).reshape(count, w64)
+    if r_bits % 64:
+        raw[:, -1] &= np.uint64((1 << (r_bits % 64)) - 1)
+    zero = ~raw.any(axis=1)
+    raw[zero, 0] = 1
+    words = raw.view(np.uint32).reshape(count, 2 * w64)
+    return np.ascontiguousarray(words[:, : (r_bits + 31) // 32])
+
+
+# ---------------------------------------------------------------------------
+# Context and keys
+# ---------------------------------------------------------------------------
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, torch.Tensor):
+        return ctypes.c_void_p(t.data_ptr())
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data_as(ctypes.c_void_p)
+    raise TypeError(type(t))
+
+
+class Context:
+    """A device plus the stream work is queued on (torch's current stream)."""
+
+    def __init__(self, device: int = 0):
+        self.device = torch.device("cuda", device)
+        h = ctypes.c_void_p()
+        check(lib().heu_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        stream = torch.cuda.current_stream(self.device)
+        check(lib().heu_ctx_set_stream(self._h, ctypes.c_void_p(stream.cuda_stream)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self) -> None:
+        check(lib().heu_ctx_sync(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and _native._lib is not None:
+            lib().heu_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def keygen(key_bits: int, seed: int | None = None) -> dict:
+    """paillier_zahlen/key_generator.cc:26-60 (seeded when seed is not None)."""
+    n = np.zeros(key_bits // 32, np.uint32)
+    hs = np.zeros(key_bits // 16, np.uint32)
+    p = np.zeros(key_bits // 64, np.uint32)
+    q = np.zeros(key_bits // 64, np.uint32)
+    sp = ctypes.byref(ctypes.c_uint64(seed)) if seed is not None else None
+    check(lib().heu_keygen(key_bits, sp, _ptr(n), _ptr(hs), _ptr(p), _ptr(q)))
+    return {"bits": key_bits, "n": words_to_int(n), "hs": words_to_int(hs), "p": words_to_int(p),
+            "q": words_to_int(q)}
+
+
+class Key:
+    """A Paillier key resident on the device (public part, optional secret part)."""
+
+    def __init__(self, ctx: Context, n: int, hs: int, p: int | None = None, q: int | None = None,
+                 r_bits: int = 256, window_bits: int = 10):
+        self.ctx = ctx
+        self.n, self.hs, self.p, self.q = n, hs, p, q
+        n_words = (n.bit_length() + 31) // 32
+        nw = int_to_words(n, n_words)
+        hw = int_to_words(hs, 2 * n_words)
+        h = ctypes.c_void_p()
+        if p is not None:
+            pw_words = (max(p.bit_length(), q.bit_length()) + 31) // 32
+            pw, qw = int_to_words(p, pw_words), int_to_words(q, pw_words)
+            check(lib().heu_key_create(ctx.handle, _ptr(nw), _ptr(hw), n_words, _ptr(pw), _ptr(qw), pw_words,
+                                       r_bits, window_bits, ctypes.byref(h)))
+        else:
+            check(lib().heu_key_create(ctx.handle, _ptr(nw), _ptr(hw), n_words, None, None, 0, r_bits,
+                                       window_bits, ctypes.byref(h)))
+        self._h = h
+        kb, nwd, cw, sk = (ctypes.c_uint32() for _ in range(4))
+        check(lib().heu_key_info(h, ctypes.byref(kb), ctypes.byref(nwd), ctypes.byref(cw), ctypes.byref(sk)))
+        self.key_bits, self.n_words, self.ct_words, self.has_sk = kb.value, nwd.value, cw.value, bool(sk.value)
+        self.r_bits = r_bits
+        self.window_bits = window_bits
+
+    @classmethod
+    def from_keygen(cls, ctx: Context, kd: dict, **kw) -> "Key":
+        return cls(ctx, kd["n"], kd["hs"], kd.get("p"), kd.get("q"), **kw)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and _native._lib is not None:
+            lib().heu_key_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().heu_key_device_bytes(self._h)
+
+    # ------------------------------------------------------------- helpers
+    def _dev(self, a, dtype) -> torch.Tensor:
+        if isinstance(a, torch.Tensor):
+            t = a.to(self.ctx.device)
+        else:
+            a = np.ascontiguousarray(a)
+            if a.dtype == np.uint32:
+                a = a.view(np.int32)
+            elif a.dtype == np.uint8:
+                pass
+            t = torch.from_numpy(a).to(self.ctx.device)
+        if t.dtype != dtype:
+            t = t.to(dtype)
+        return t.contiguous()
+
+    def _ct_out(self, count: int) -> torch.Tensor:
+        return torch.empty((count, self.ct_words), dtype=torch.int32, device=self.ctx.device)
+
+    def _rows(self, t: torch.Tensor) -> int:
+        if t.dim() != 2 or t.shape[1] != self.ct_words:
+            raise ValueError(f"ciphertexts must have shape (N, {self.ct_words})")
+        return t.shape[0]
+
+    # ------------------------------------------------------------- ops
+    def encrypt_i64(self, m, r) -> torch.Tensor:
+        m = self._dev(m, torch.int64)
+        r = self._dev(r, torch.int32)
+        if r.dim() != 2 or r.shape[0] != m.shape[0]:
+            raise ValueError("r must have shape (N, r_words)")
+        out = self._ct_out(m.shape[0])
+        check(lib().heu_encrypt_i64(self._h, _ptr(m), _ptr(r), r.shape[1], m.shape[0], _ptr(out)))
+        return out
+
+    def encrypt(self, values, r) -> torch.Tensor:
+        """Arbitrary Python ints (|m| <= n/2)."""
+        mw = self.n_words
+        mh = ints_to_words(values, mw, signed=True)
+        md = self._dev(mh, torch.int32)
+        r = self._dev(r, torch.int32)
+        out = self._ct_out(len(values))
+        check(lib().heu_encrypt(self._h, _ptr(md), _ptr(mh), mw, _ptr(r), r.shape[1], len(values), _ptr(out)))
+        return out
+
+    def randomize(self, ct, r) -> torch.Tensor:
+        r = self._dev(r, torch.int32)
+        if ct is None:
+            n = r.shape[0]
+            out = self._ct_out(n)
+            check(lib().heu_randomize(self._h, None, _ptr(r), r.shape[1], n, _ptr(out)))
+            return out
+        ct = self._dev(ct, torch.int32)
+        n = self._rows(ct)
+        out = self._ct_out(n)
+        check(lib().heu_randomize(self._h, _ptr(ct), _ptr(r), r.shape[1], n, _ptr(out)))
+        return out
+
+    def decrypt(self, ct, out_words: int | None = None, overflow: bool = False):
+        ct = self._dev(ct, torch.int32)
+        n = self._rows(ct)
+        ow = out_words or self.n_words
+        out = torch.empty((n, ow), dtype=torch.int32, device=self.ctx.device)
+        ovf = torch.empty((n,), dtype=torch.uint8, device=self.ctx.device) if overflow else None
+        check(lib().heu_decrypt(self._h, _ptr(ct), n, _ptr(out), ow, _ptr(ovf)))
+        return (out, ovf) if overflow else out
+
+    def add(self, a, b) -> torch.Tensor:
+        a, b = self._dev(a, torch.int32), self._dev(b, torch.int32)
+        n = self._rows(a)
+        out = self._ct_out(n)
+        check(lib().heu_add(self._h, _ptr(a), _ptr(b), n, _ptr(out)))
+        return out
+
+    def add_i64(self, a, p) -> torch.Tensor:
+        a, p = self._dev(a, torch.int32), self._dev(p, torch.int64)
+        n = self._rows(a)
+        out = self._ct_out(n)
+        check(lib().heu_add_i64(self._h, _ptr(a), _ptr(p), n, _ptr(out)))
+        return out
+
+    def sub(self, a, b) -> torch.Tensor:
+        a, b = self._dev(a, torch.int32), self._dev(b, torch.int32)
+        n = self._rows(a)
+        out = self._ct_out(n)
+        check(lib().heu_sub(self._h, _ptr(a), _ptr(b), n, _ptr(out)))
+        return out
+
+    def negate(self, a) -> torch.Tensor:
+        a = self._dev(a, torch.int32)
+        n = self._rows(a)
+        out = self._ct_out(n)
+        check(lib().heu_negate(self._h, _ptr(a), n, _ptr(out)))
+        return out
+
+    def mul_i64(self, a, k) -> torch.Tensor:
+        a, k = self._dev(a, torch.int32), self._dev(k, torch.int64)
+        n = self._rows(a)
+        out = self._ct_out(n)
+        check(lib().heu_mul_i64(self._h, _ptr(a), _ptr(k), n, _ptr(out)))
+        return out
+
+    def sum(self, ct, offsets) -> torch.Tensor:
+        ct = self._dev(ct, torch.int32)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        segs = len(offs) - 1
+        out = self._ct_out(segs)
+        check(lib().heu_sum(self._h, _ptr(ct), _ptr(offs), segs, _ptr(out)))
+        return out
+
+    def dot_i64(self, ct, w, offsets) -> torch.Tensor:
+        ct, w = self._dev(ct, torch.int32), self._dev(w, torch.int64)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        segs = len(offs) - 1
+        out = self._ct_out(segs)
+        check(lib().heu_dot_i64(self._h, _ptr(ct), _ptr(w), _ptr(offs), segs, _ptr(out)))
+        return out
+
+    def to_mont(self, ct, word_bits: int = 64) -> torch.Tensor:
+        ct = self._dev(ct, torch.int32)
+        n = self._rows(ct)
+        out = self._ct_out(n)
+        check(lib().heu_to_mont(self._h, _ptr(ct), n, word_bits, _ptr(out)))
+        return out
+
+    def from_mont(self, ct, word_bits: int = 64) -> torch.Tensor:
+        ct = self._dev(ct, torch.int32)
+        n = self._rows(ct)
+        out = self._ct_out(n)
+        check(lib().heu_from_mont(self._h, _ptr(ct), n, word_bits, _ptr(out)))
+        return out
+
+
+def to_host_words(t: torch.Tensor) -> np.ndarray:
+    return t.cpu().numpy().view(np.uint32)

@@ -1,0 +1,128 @@
+/*
+ * heu_paillier.h -- C ABI of the MI355X Paillier batch engine.
+ *
+ * This replaces the C ABI that HEU's CUDA backend binds,
+ * heu/library/algorithms/paillier_gpu/gpulib/gpupaillier.h:17-91 (one
+ * synchronous `int gpu_paillier_<op>(...)` per op over 512-byte host structs,
+ * device 0 only, allocation per call).  Differences by design:
+ *   - a context (device + stream) and resident keys are explicit handles; keys
+ *     keep n, n^2, Montgomery constants, CRT constants and the fixed-base table
+ *     of h_s on the device;
+ *   - batch entry points take DEVICE pointers to flat arrays of little-endian
+ *     uint32 limbs and are asynchronous on the context's stream;
+ *     heu_ctx_sync() waits and reports device-side errors;
+ *   - ciphertexts are the Z-space value c = (1 + n*m) * h_s^r mod n^2 of
+ *     heu/library/algorithms/paillier_zahlen/encryptor.cc:42-49, stored as
+ *     ct_words = 2*n_words limbs (128 for 2048-bit keys, 192 for 3072-bit).
+ *     heu_to_mont()/heu_from_mont() convert to the Montgomery form that
+ *     paillier_z::Ciphertext::c_ stores, for a stated word size.
+ *   - every entry point returns a status; heu_last_error() gives the message.
+ *
+ * Signed plaintexts cross as two's complement over a stated number of limbs.
+ */
+#ifndef HEU_PAILLIER_H_
+#define HEU_PAILLIER_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct heu_ctx heu_ctx;
+typedef struct heu_key heu_key;
+
+enum {
+  HEU_OK = 0,
+  HEU_EINVAL = 1,       /* bad argument */
+  HEU_ERANGE = 2,       /* plaintext / randomness / output out of range */
+  HEU_ENOTINV = 3,      /* a value has no inverse mod n^2 */
+  HEU_ENOMEM = 4,       /* device allocation failed */
+  HEU_EDEVICE = 5,      /* HIP runtime error */
+  HEU_EUNSUPPORTED = 6, /* key size without kernels */
+  HEU_EKEY = 7          /* pk and sk are not paired / no secret key */
+};
+
+/* Message of the last failing call on this thread. */
+const char *heu_last_error(void);
+
+/* ---- context ---------------------------------------------------------- */
+/* Creates a context on HIP device `device` with its own stream. */
+int heu_ctx_create(int device, heu_ctx **out);
+int heu_ctx_destroy(heu_ctx *ctx);
+/* Makes subsequent work run on `stream` (a hipStream_t), e.g. the caller's
+ * framework stream.  NULL restores the context's own stream. */
+int heu_ctx_set_stream(heu_ctx *ctx, void *stream);
+void *heu_ctx_stream(heu_ctx *ctx);
+/* Waits for all enqueued work; returns HEU_ERANGE / HEU_ENOTINV if a kernel
+ * flagged bad input since the last sync, then clears the flags. */
+int heu_ctx_sync(heu_ctx *ctx);
+
+/* ---- key generation (paillier_zahlen/key_generator.cc:26-60) ------------ */
+/* Writes n (key_bits/32 limbs), h_s (key_bits/16), p and q (key_bits/64 each).
+ * seed == NULL draws from the OS RNG like the reference; otherwise every
+ * random word comes from a counter stream keyed by *seed, so a seed always
+ * gives the same key. */
+int heu_keygen(uint32_t key_bits, const uint64_t *seed, uint32_t *n, uint32_t *h_s, uint32_t *p, uint32_t *q);
+
+/* ---- keys (paillier_zahlen/public_key.cc:28-40, secret_key.cc:19-62) ---- */
+/* n: n_words limbs; h_s: 2*n_words limbs.  p, q (prime_words limbs each) may
+ * be NULL for a public-only key.  r_bits: largest bit length of the
+ * randomness r the fixed-base table must cover (the reference draws
+ * key_bits/2 bits, encryptor.cc:27); window_bits: table window (the
+ * reference's kExpUnitBits = 10, public_key.cc:21). */
+int heu_key_create(heu_ctx *ctx, const uint32_t *n, const uint32_t *h_s, uint32_t n_words, const uint32_t *p,
+                   const uint32_t *q, uint32_t prime_words, uint32_t r_bits, uint32_t window_bits, heu_key **out);
+int heu_key_destroy(heu_key *key);
+int heu_key_info(const heu_key *key, uint32_t *key_bits, uint32_t *n_words, uint32_t *ct_words, uint32_t *has_sk);
+/* Device bytes held by the key (constants + table). */
+size_t heu_key_device_bytes(const heu_key *key);
+
+/* ---- batch operations: device pointers, asynchronous ------------------- */
+/* Encryptor::Encrypt (paillier_zahlen/encryptor.cc:35-49) with injected r:
+ * ct[i] = (1 + n*m[i]) * h_s^r[i] mod n^2.  r: count x r_words limbs. */
+int heu_encrypt_i64(heu_key *key, const int64_t *m, const uint32_t *r, uint32_t r_words, size_t count,
+                    uint32_t *ct);
+/* Same for plaintexts given as two's complement over m_words limbs; |m| <= n/2
+ * (checked on the host copy `m_host`, which must hold the same values). */
+int heu_encrypt(heu_key *key, const uint32_t *m, const uint32_t *m_host, uint32_t m_words, const uint32_t *r,
+                uint32_t r_words, size_t count, uint32_t *ct);
+/* Evaluator::Randomize (evaluator.cc:25-28): out = ct * h_s^r.  ct == NULL
+ * gives Encryptor::EncryptZero (encryptor.cc:33). */
+int heu_randomize(heu_key *key, const uint32_t *ct, const uint32_t *r, uint32_t r_words, size_t count,
+                  uint32_t *out);
+/* Decryptor::Decrypt (decryptor.cc:32-53), CRT.  out: count x out_words
+ * limbs, the plaintext sign-extended or truncated to out_words; overflow[i]
+ * (may be NULL) is set when it did not fit. */
+int heu_decrypt(heu_key *key, const uint32_t *ct, size_t count, uint32_t *out, uint32_t out_words, uint8_t *overflow);
+/* Evaluator::Add(ct, ct) (evaluator.cc:30-37): out = a * b mod n^2. */
+int heu_add(heu_key *key, const uint32_t *a, const uint32_t *b, size_t count, uint32_t *out);
+/* Evaluator::Add(ct, pt) (evaluator.cc:46-64): out = ct * (1 + n*pt). */
+int heu_add_i64(heu_key *key, const uint32_t *ct, const int64_t *pt, size_t count, uint32_t *out);
+/* Evaluator::Sub(ct, ct) (evaluator.cc:66-68): out = a * b^-1 mod n^2. */
+int heu_sub(heu_key *key, const uint32_t *a, const uint32_t *b, size_t count, uint32_t *out);
+/* Evaluator::Negate (evaluator.cc:82-91): out = a^-1 mod n^2. */
+int heu_negate(heu_key *key, const uint32_t *a, size_t count, uint32_t *out);
+/* Evaluator::Mul(ct, p) (evaluator.cc:99-123): out = ct^k mod n^2, k = 0 ->
+ * 1, k = 1 -> ct, k < 0 -> inverse power. */
+int heu_mul_i64(heu_key *key, const uint32_t *ct, const int64_t *k, size_t count, uint32_t *out);
+/* numpy Evaluator::Sum per segment (numpy/evaluator.cc:355-373): out[s] =
+ * prod ct[offsets[s] .. offsets[s+1]) mod n^2 (1 for an empty segment).
+ * offsets: HOST array of segments+1 entries. */
+int heu_sum(heu_key *key, const uint32_t *ct, const int64_t *offsets, size_t segments, uint32_t *out);
+/* MatMul / dot per segment (numpy/evaluator.cc:209-292): out[s] =
+ * prod ct[i]^w[i] over the segment.  w: device array, one weight per ct. */
+int heu_dot_i64(heu_key *key, const uint32_t *ct, const int64_t *w, const int64_t *offsets, size_t segments,
+                uint32_t *out);
+/* Montgomery-form export / import for R = 2^(word_bits*ceil(bits(n^2)/word_bits))
+ * (MontgomerySpace::MapIntoMSpace / MapBackToZSpace, encryptor.cc:46,
+ * decryptor.cc:36). */
+int heu_to_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
+int heu_from_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEU_PAILLIER_H_ */

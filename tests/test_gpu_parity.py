@@ -1,0 +1,206 @@
+"""GPU parity: every engine entry point against the CPU oracle (a restatement of
+paillier_zahlen over OpenSSL BIGNUM) on the same seeded keys and inputs.
+
+Bit-exact comparisons of Z-space ciphertexts and of decrypted plaintexts.
+Mirrors the reference's own cases: paillier_zahlen/paillier_test.cc:40-233
+(values {-12345, 123, +-1, +-2, 0, INT64 extremes}, Mul by {2, 1, 0, -1, -2}),
+paillier_gpu/paillier_gpu_test.cc:84-165 (EncDec / Add with int64 extremes).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+I64_MIN, I64_MAX = -(2**63), 2**63 - 1
+EDGE_I64 = [I64_MIN, I64_MIN + 1, -12345, -1234, -2, -1, 0, 1, 2, 123, 1234, I64_MAX]
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from heu_amd import engine
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return engine
+
+
+@pytest.fixture(scope="module")
+def ctx(eng):
+    return eng.Context(0)
+
+
+def _keys(eng, oracle_mod, ctx, bits, seed):
+    kd = eng.keygen(bits, seed)
+    gk = eng.Key.from_keygen(ctx, kd, r_bits=256, window_bits=10)
+    ok = oracle_mod.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
+    return kd, gk, ok
+
+
+@pytest.fixture(scope="module")
+def k2048(eng, oracle_mod, ctx):
+    return _keys(eng, oracle_mod, ctx, 2048, 0)
+
+
+@pytest.fixture(scope="module")
+def k3072(eng, oracle_mod, ctx):
+    return _keys(eng, oracle_mod, ctx, 3072, 1)
+
+
+def host(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def inputs(eng, n, seed=7):
+    rng = np.random.default_rng(seed)
+    m = rng.integers(I64_MIN, I64_MAX, size=n, dtype=np.int64, endpoint=True)
+    m[: len(EDGE_I64)] = EDGE_I64[:n]
+    r = eng.seeded_r(seed, 11, n, 256)
+    return m, r
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_encrypt_decrypt_i64(eng, which, request):
+    kd, gk, ok = request.getfixturevalue(which)
+    m, r = inputs(eng, 96)
+    print(f"[{which}] encrypt_i64 n=96", flush=True)
+    ct = gk.encrypt_i64(m, r)
+    gk.ctx.sync()
+    ref = ok.encrypt_i64(m, r)
+    assert np.array_equal(host(ct), ref)
+    print(f"[{which}] decrypt n=96", flush=True)
+    pt = gk.decrypt(ct, out_words=2)
+    gk.ctx.sync()
+    assert np.array_equal(host(pt).view(np.int64).ravel(), m)
+    full = gk.decrypt(ct)
+    gk.ctx.sync()
+    assert np.array_equal(host(full), ok.decrypt(ref))
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_add_and_add_pt(eng, which, request):
+    kd, gk, ok = request.getfixturevalue(which)
+    m, r = inputs(eng, 64, 3)
+    m2, r2 = inputs(eng, 64, 4)
+    a, b = ok.encrypt_i64(m, r), ok.encrypt_i64(m2, r2)
+    print(f"[{which}] add n=64", flush=True)
+    s = gk.add(a, b)
+    gk.ctx.sync()
+    assert np.array_equal(host(s), ok.add(a, b))
+    dec = ok.decrypt(host(s), out_words=3)
+    want = [int(x) + int(y) for x, y in zip(m, m2)]
+    assert [eng.twos_to_int(row) for row in dec] == want
+    print(f"[{which}] add_i64 n=64", flush=True)
+    s2 = gk.add_i64(a, m2)
+    gk.ctx.sync()
+    assert np.array_equal(host(s2), ok.add_pt_i64(a, m2))
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_mul_scalar(eng, which, request):
+    kd, gk, ok = request.getfixturevalue(which)
+    m, r = inputs(eng, 48, 5)
+    a = ok.encrypt_i64(m, r)
+    rng = np.random.default_rng(9)
+    k = rng.integers(-(2**31), 2**31 - 1, size=48, dtype=np.int64)
+    k[:8] = [0, 1, -1, 2, -2, 3, I64_MAX, I64_MIN + 1]
+    print(f"[{which}] mul_i64 n=48", flush=True)
+    out = gk.mul_i64(a, k)
+    gk.ctx.sync()
+    assert np.array_equal(host(out), ok.mul_i64(a, k))
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_negate_sub(eng, which, request):
+    kd, gk, ok = request.getfixturevalue(which)
+    m, r = inputs(eng, 40, 6)
+    m2, r2 = inputs(eng, 40, 8)
+    a, b = ok.encrypt_i64(m, r), ok.encrypt_i64(m2, r2)
+    print(f"[{which}] negate n=40", flush=True)
+    ng = gk.negate(a)
+    gk.ctx.sync()
+    assert np.array_equal(host(ng), ok.negate(a))
+    print(f"[{which}] sub n=40", flush=True)
+    sb = gk.sub(a, b)
+    gk.ctx.sync()
+    assert np.array_equal(host(sb), ok.sub(a, b))
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_sum_dot_segments(eng, which, request):
+    kd, gk, ok = request.getfixturevalue(which)
+    n = 150
+    m, r = inputs(eng, n, 10)
+    a = ok.encrypt_i64(m, r)
+    offsets = np.array([0, 0, 1, 3, 40, 41, 41, 120, 150], dtype=np.int64)  # empty, 1, 2, 37, ...
+    print(f"[{which}] sum segments={len(offsets) - 1}", flush=True)
+    s = gk.sum(a, offsets)
+    gk.ctx.sync()
+    assert np.array_equal(host(s), ok.dot_i64(a, None, offsets))
+    w = np.random.default_rng(11).integers(-(2**31), 2**31 - 1, size=n, dtype=np.int64)
+    w[:6] = [0, 1, -1, 2, -2, 0]
+    print(f"[{which}] dot segments={len(offsets) - 1}", flush=True)
+    d = gk.dot_i64(a, w, offsets)
+    gk.ctx.sync()
+    assert np.array_equal(host(d), ok.dot_i64(a, w, offsets))
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_encrypt_bigint_and_bounds(eng, which, request):
+    kd, gk, ok = request.getfixturevalue(which)
+    n_half = kd["n"] >> 1
+    vals = [n_half, -n_half, n_half - 1, -(n_half - 12345), 2**200 + 7, -(2**1000), 0, 1, -1]
+    r = eng.seeded_r(12, 11, len(vals), 256)
+    mw = gk.n_words
+    print(f"[{which}] encrypt big n={len(vals)}", flush=True)
+    ct = gk.encrypt(vals, r)
+    gk.ctx.sync()
+    ref = ok.encrypt_twos(eng.ints_to_words(vals, mw, signed=True), r)
+    assert np.array_equal(host(ct), ref)
+    dec = gk.decrypt(ct)
+    gk.ctx.sync()
+    assert eng.words_to_ints(host(dec), signed=True) == vals
+    from heu_amd._native import HeuError
+
+    with pytest.raises(HeuError):
+        gk.encrypt([n_half + 1], r[:1])
+
+
+def test_mont_export_and_randomize(eng, k2048):
+    kd, gk, ok = k2048
+    m, r = inputs(eng, 32, 13)
+    a = ok.encrypt_i64(m, r)
+    print("[k2048] to_mont / from_mont", flush=True)
+    tm = gk.to_mont(a, 64)
+    gk.ctx.sync()
+    assert np.array_equal(host(tm), ok.to_mont(a, 64))
+    back = gk.from_mont(tm, 64)
+    gk.ctx.sync()
+    assert np.array_equal(host(back), a)
+    print("[k2048] randomize / encrypt zero", flush=True)
+    r2 = eng.seeded_r(14, 11, 32, 256)
+    rz = gk.randomize(a, r2)
+    gk.ctx.sync()
+    assert np.array_equal(host(rz), ok.randomize(a, r2))
+    z = gk.randomize(None, r2)
+    gk.ctx.sync()
+    assert np.array_equal(host(z), ok.encrypt_i64(np.zeros(32, np.int64), r2))
+
+
+def test_errors(eng, ctx, k2048):
+    from heu_amd._native import HeuError
+
+    kd, gk, ok = k2048
+    pub = eng.Key(ctx, kd["n"], kd["hs"])
+    m, r = inputs(eng, 4, 15)
+    ct = pub.encrypt_i64(m, r)
+    ctx.sync()
+    with pytest.raises(HeuError):
+        pub.decrypt(ct)
+    big_r = np.zeros((4, 9), np.uint32)
+    big_r[:, 8] = 1 << 4  # r = 2^260: beyond the 26 x 10-bit table
+    pub.encrypt_i64(m, big_r)
+    with pytest.raises(HeuError):
+        ctx.sync()
+    ctx.sync()  # flags cleared
