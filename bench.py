@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Paillier-2048 batch benchmark on one MI355X.
+
+Workload (one op per launch, 1,048,576 elements): a step encrypts int64-uniform
+plaintexts with seeded r < 2^256, adds the result to a resident ciphertext
+vector (ct+ct), multiplies by int32-uniform scalars (ct x pt, about half
+negative, so the batch inversion runs) and CRT-decrypts.  Inputs are resident in
+HBM before the timed region.  `value` is elements per second through the whole
+chain (enc + add + ct*pt + dec per element); per-op rates are reported beside it.
+
+The JSON line also carries:
+  roofline      the dominant kernel's algorithmic 32x32->64 multiply-add rate
+                (M(s) = 2 s^2 + s per Montgomery product over s 32-bit words),
+                timed live with HIP events on the work stream, against the
+                v_mad_u64_u32 issue peak (64 lanes/clk/CU x 256 CUs x 2.4 GHz);
+  cpu_baseline  the CPU oracle (paillier_zahlen restated over OpenSSL BIGNUM,
+                fixed-base w=10 table for encryption, CRT decryption) on a
+                bounded sample of the same inputs on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Paillier-2048 enc/s, dec/s, ct*pt/s on 1 MI355X; served p50/p99 ms"
+
+
+def M(s: int) -> int:
+    return 2 * s * s + s
+
+
+# Algorithmic multiply-adds per element (2048-bit keys: n^2 = 128 words, p^2 = 64 words).
+ALGO_MACS = {
+    "encrypt_i64": 28 * M(128),             # ceil(256/10) + 2 products, r < 2^256
+    "mulmod": M(128),                       # ct + ct
+    "pow_i64": 44 * M(128),                 # ct x int32, sliding window
+    "dec_half": 2 * (1023 + 171 + 16) * M(64),  # both CRT exponentiations per element
+}
+VALU_MAC_PEAK_T = 64 * 256 * 2.4e9 / 1e12   # 39.32 T MAC/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--elements", type=int, default=1 << 20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (s)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device-to-device copy kernel (HBM counter calibration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(kd, m, r, k, budget_s):
+    """Times the oracle on this host: all cores (capped at 16) on a bounded sample."""
+    from oracle import oracle as O
+
+    O.build()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    key = O.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
+    key.build_table(10, 256)
+    per_op = budget_s / 4.0
+    rates, samples = {}, {}
+
+    def run(name, fn, pilot):
+        t = time.perf_counter()
+        fn(pilot)
+        rate = pilot / max(time.perf_counter() - t, 1e-6)
+        n = int(min(len(m), max(pilot, rate * per_op)))
+        t = time.perf_counter()
+        fn(n)
+        rates[name] = n / (time.perf_counter() - t)
+        samples[name] = n
+
+    ct_cache = {}
+
+    def enc(n):
+        ct_cache["ct"] = key.encrypt_i64(m[:n], r[:n], use_table=True, threads=threads)
+
+    run("enc", enc, 4 * threads)
+    n_ct = len(ct_cache["ct"])
+    ctb = ct_cache["ct"]
+
+    def ensure(n):
+        nonlocal ctb
+        if len(ctb) < n:
+            ctb = key.encrypt_i64(m[:n], r[:n], use_table=True, threads=threads)
+        return ctb[:n]
+
+    run("add", lambda n: key.add(ensure(n), ensure(n)[::-1].copy(), threads=threads), 64 * threads)
+    run("ctpt", lambda n: key.mul_i64(ensure(n), k[:n], threads=threads), 2 * threads)
+    run("dec", lambda n: key.decrypt(ensure(n), out_words=2, threads=threads), threads)
+    chain = 1.0 / sum(1.0 / rates[x] for x in ("enc", "add", "ctpt", "dec"))
+    _ = n_ct
+    return {
+        "value": chain,
+        "unit": "elements/s (enc+add+ct*pt+dec)",
+        "cores": threads,
+        "kind": "port",
+        "sample": ", ".join(f"{x} {samples[x]} elts" for x in ("enc", "add", "ctpt", "dec"))
+        + f" of the same workload, {threads} threads (std::thread-style chunking per op)",
+        "per_op": {x: rates[x] for x in rates},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this box (every run here "
+              "uses a single MI355X; see DESIGN.md for how the path shards).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from heu_amd import engine as E
+
+    N = args.elements
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)  # 256 MiB read + 256 MiB written by one copy kernel
+        torch.cuda.synchronize()
+        del src, dst
+    ctx = E.Context(0)
+    kd = E.keygen(2048, 0)
+    key = E.Key.from_keygen(ctx, kd, r_bits=256, window_bits=10)
+    rng = np.random.default_rng(0)
+    m_h = rng.integers(-(2**63), 2**63 - 1, size=N, dtype=np.int64, endpoint=True)
+    r_h = E.seeded_r(0, 11, N, 256)
+    k_h = rng.integers(-(2**31), 2**31 - 1, size=N, dtype=np.int64, endpoint=True)
+    m = torch.from_numpy(m_h).cuda()
+    r = torch.from_numpy(r_h.view(np.int32)).cuda()
+    k = torch.from_numpy(k_h).cuda()
+    ct_res = key.encrypt_i64(m.flip(0).contiguous(), r)  # resident second operand
+    ctx.sync()
+
+    ops = ("enc", "add", "ctpt", "dec")
+
+    def step(ev=None):
+        marks = []
+        def mark():
+            if ev is not None:
+                e = torch.cuda.Event(enable_timing=True)
+                e.record()
+                marks.append(e)
+        mark()
+        ct = key.encrypt_i64(m, r)
+        mark()
+        s = key.add(ct, ct_res)
+        mark()
+        p = key.mul_i64(s, k)
+        mark()
+        d = key.decrypt(p, out_words=4)
+        mark()
+        if ev is not None:
+            ev.append(marks)
+        return ct, s, p, d
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+
+    ctx.profile(True)
+    evs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step(evs)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.sync()
+    # the chain's result, checked on the host: dec((enc(m) (+) enc(m')) (x) k) = (m + m') k
+    d = out[3][:4096].cpu().numpy().view(np.int64).reshape(-1, 2)
+    got = [int(a) % (1 << 64) + (int(b) << 64) for a, b in zip(d[:, 0], d[:, 1])]
+    nchk = len(got)
+    want = [(int(a) + int(b)) * int(c) for a, b, c in zip(m_h[:nchk], m_h[::-1][:nchk], k_h[:nchk])]
+    chain_ok = got == want
+    per_op_ms = {op: 0.0 for op in ops}
+    for marks in evs:
+        for i, op in enumerate(ops):
+            per_op_ms[op] += marks[i].elapsed_time(marks[i + 1])
+    per_op_rate = {op: N * args.steps / (per_op_ms[op] / 1e3) for op in ops}
+    kern = {}
+    for name in ALGO_MACS:
+        total, launches = ctx.profile_read(name)
+        if launches:
+            kern[name] = (total, launches)
+    ctx.profile(False)
+    dom = max(kern, key=lambda n: kern[n][0])
+    dom_ms, dom_launches = kern[dom]
+    elems_per_launch = (N * args.steps) / dom_launches
+    macs_per_launch = ALGO_MACS[dom] * elems_per_launch
+    avg_s = dom_ms / dom_launches / 1e3
+    achieved = macs_per_launch / avg_s / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(dom)
+        except Exception:
+            traffic = None
+
+    ms_per_step = elapsed / args.steps * 1e3
+    value = N / (elapsed / args.steps)
+    cpu = None
+    if not args.no_cpu:
+        n_cpu = min(N, 1 << 16)
+        cpu = cpu_baseline(kd, m_h[:n_cpu], r_h[:n_cpu], k_h[:n_cpu], args.cpu_seconds)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "elements/s (enc+add+ct*pt+dec per element)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded int64 plaintexts, r < 2^256, int32 scalars)",
+        "config": {
+            "workload": "paillier2048 one-op-per-launch: enc -> ct+ct -> ct*int32 -> CRT dec",
+            "elements": N,
+            "key_bits": 2048,
+            "r_bits": 256,
+            "table_window_bits": 10,
+            "parallelism": "1 GPU",
+        },
+        "per_op_per_s": {"enc": per_op_rate["enc"], "add": per_op_rate["add"], "ctpt": per_op_rate["ctpt"],
+                         "dec": per_op_rate["dec"]},
+        "chain_check": chain_ok,
+        "served_p50_ms": None,
+        "served_p99_ms": None,
+        "roofline": {
+            "bound": "valu",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": VALU_MAC_PEAK_T,
+            "unit": "T MAC/s (32x32->64 multiply-add, v_mad_u64_u32)",
+            "frac": achieved / VALU_MAC_PEAK_T,
+            "traffic": traffic,
+            "avg_launch_ms": dom_ms / dom_launches,
+            "launches": dom_launches,
+            "algorithmic_macs_per_element": ALGO_MACS[dom],
+        },
+        "kernel_ms": {n: kern[n][0] for n in kern},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["cpu_baseline"]["speedup_gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

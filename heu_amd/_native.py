@@ -23,6 +23,8 @@ SIGNATURES = {
     "heu_ctx_set_stream": (I, [P, P]),
     "heu_ctx_stream": (P, [P]),
     "heu_ctx_sync": (I, [P]),
+    "heu_ctx_profile": (I, [P, I]),
+    "heu_ctx_profile_read": (I, [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)]),
     "heu_keygen": (I, [U32, ctypes.POINTER(U64), P, P, P, P]),
     "heu_key_create": (I, [P, P, P, U32, P, P, U32, U32, U32, ctypes.POINTER(P)]),
     "heu_key_destroy": (I, [P]),

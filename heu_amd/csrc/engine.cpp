@@ -180,7 +180,48 @@ struct heu_ctx {
   size_t scratch_used = 0;
   // small host->device staging for chunk lists etc.
   DevBuf meta;
+  // optional per-kernel timing: HIP events recorded on the work stream
+  struct ProfRec {
+    const char* name;
+    hipEvent_t start, stop;
+  };
+  bool prof = false;
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;
+  hipEvent_t prof_event() {
+    if (!prof_pool.empty()) {
+      hipEvent_t e = prof_pool.back();
+      prof_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void prof_clear() {
+    for (auto& r : prof_recs) {
+      prof_pool.push_back(r.start);
+      prof_pool.push_back(r.stop);
+    }
+    prof_recs.clear();
+  }
 };
+
+// Launches `call` and, when profiling is on, brackets it with events.
+#define LAUNCH(ctx, name, call)                             \
+  do {                                                      \
+    hipEvent_t a_ = nullptr;                                \
+    if ((ctx)->prof) {                                      \
+      a_ = (ctx)->prof_event();                             \
+      (void)hipEventRecord(a_, (ctx)->stream);              \
+    }                                                       \
+    call;                                                   \
+    if ((ctx)->prof) {                                      \
+      hipEvent_t b_ = (ctx)->prof_event();                  \
+      (void)hipEventRecord(b_, (ctx)->stream);              \
+      (ctx)->prof_recs.push_back({(name), a_, b_});         \
+    }                                                       \
+  } while (0)
 
 struct heu_key {
   heu_ctx* ctx = nullptr;
@@ -331,18 +372,18 @@ int batch_invert(heu_key* key, const uint32_t* in, const uint8_t* sel, long coun
   // up: level l reduces cnt[l] values into cnt[l+1] totals
   for (int l = 0; l < levels; ++l) {
     const uint32_t* src = l == 0 ? in : sc.at<uint32_t>(tot[l]);
-    o->inv_prefix(st, mod, l == 0, src, l == 0 ? sel : nullptr, cnt[l], kInvChunk, sc.at<uint32_t>(pre[l]),
-                  sc.at<uint32_t>(tot[l + 1]));
+    LAUNCH(ctx, "inv_prefix", o->inv_prefix(st, mod, l == 0, src, l == 0 ? sel : nullptr, cnt[l], kInvChunk, sc.at<uint32_t>(pre[l]),
+                  sc.at<uint32_t>(tot[l + 1])));
   }
   // the single top value
   HIP_TRY(hipMemcpyAsync(sc.at<uint32_t>(inv[levels]), sc.at<uint32_t>(tot[levels]), NLB, hipMemcpyDeviceToDevice, st));
-  o->inv_single(st, mod, key->n2_words.as<uint32_t>(), sc.at<uint32_t>(inv[levels]), ctx->d_err);
+  LAUNCH(ctx, "inv_single", o->inv_single(st, mod, key->n2_words.as<uint32_t>(), sc.at<uint32_t>(inv[levels]), ctx->d_err));
   // down
   for (int l = levels - 1; l >= 0; --l) {
     const uint32_t* src = l == 0 ? in : sc.at<uint32_t>(tot[l]);
     uint32_t* dst = l == 0 ? out : sc.at<uint32_t>(inv[l]);
-    o->inv_down(st, mod, l == 0, src, l == 0 ? sel : nullptr, cnt[l], kInvChunk, sc.at<uint32_t>(pre[l]),
-                sc.at<uint32_t>(inv[l + 1]), dst);
+    LAUNCH(ctx, "inv_down", o->inv_down(st, mod, l == 0, src, l == 0 ? sel : nullptr, cnt[l], kInvChunk, sc.at<uint32_t>(pre[l]),
+                sc.at<uint32_t>(inv[l + 1]), dst));
   }
   HIP_TRY(hipGetLastError());
   return HEU_OK;
@@ -430,19 +471,19 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
     if ((rc = upload_meta(ctx, sc, l.ch_off, l.ch.data(), l.ch.size() * sizeof(Chunk)))) return rc;
   // level 0
   if (dot)
-    o->reduce_dot0(st, mod, ct, w, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off),
-                   sc.at<uint32_t>(lv[0].neg_off), sc.at<uint32_t>(tab_off));
+    LAUNCH(ctx, "reduce_dot0", o->reduce_dot0(st, mod, ct, w, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off),
+                   sc.at<uint32_t>(lv[0].neg_off), sc.at<uint32_t>(tab_off)));
   else
-    o->reduce_sum(st, mod, true, ct, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off));
+    LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, true, ct, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off)));
   for (size_t i = 1; i < lv.size(); ++i) {
     if (dot) {
-      o->reduce_mont(st, mod, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off), (long)lv[i].ch.size(),
-                     sc.at<uint32_t>(lv[i].pos_off));
-      o->reduce_mont(st, mod, sc.at<uint32_t>(lv[i - 1].neg_off), sc.at<Chunk>(lv[i].ch_off), (long)lv[i].ch.size(),
-                     sc.at<uint32_t>(lv[i].neg_off));
+      LAUNCH(ctx, "reduce_mont", o->reduce_mont(st, mod, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off), (long)lv[i].ch.size(),
+                     sc.at<uint32_t>(lv[i].pos_off)));
+      LAUNCH(ctx, "reduce_mont", o->reduce_mont(st, mod, sc.at<uint32_t>(lv[i - 1].neg_off), sc.at<Chunk>(lv[i].ch_off), (long)lv[i].ch.size(),
+                     sc.at<uint32_t>(lv[i].neg_off)));
     } else {
-      o->reduce_sum(st, mod, false, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off),
-                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off));
+      LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, false, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off),
+                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off)));
     }
   }
   // the last level has one chunk per segment, in segment order
@@ -451,15 +492,15 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
   if (!dot) {
     std::vector<int64_t> nt(len.begin(), len.end());
     if ((rc = upload_meta(ctx, sc, nt_off, nt.data(), segments * 8))) return rc;
-    o->seg_finish(st, mod, sc.at<uint32_t>(top_pos), sc.at<int64_t>(nt_off), (long)segments, 0, out);
+    LAUNCH(ctx, "seg_finish", o->seg_finish(st, mod, sc.at<uint32_t>(top_pos), sc.at<int64_t>(nt_off), (long)segments, 0, out));
   } else {
     HIP_TRY(
         hipMemcpyAsync(sc.at<char>(top_neg), sc.at<char>(top.neg_off), segments * NLB, hipMemcpyDeviceToDevice, st));
     // N -> Z-space, invert all N in one batch, result = P * N^-1
     uint32_t* nz = sc.at<uint32_t>(nz_off);
-    o->seg_finish(st, mod, sc.at<uint32_t>(top_neg), nullptr, (long)segments, 1, nz);
+    LAUNCH(ctx, "seg_finish", o->seg_finish(st, mod, sc.at<uint32_t>(top_neg), nullptr, (long)segments, 1, nz));
     if ((rc = batch_invert(key, nz, nullptr, (long)segments, nz, inv_base))) return rc;
-    o->mont_times_z(st, mod, sc.at<uint32_t>(top_pos), nz, (long)segments, out);
+    LAUNCH(ctx, "mont_times_z", o->mont_times_z(st, mod, sc.at<uint32_t>(top_pos), nz, (long)segments, out));
   }
   HIP_TRY(hipGetLastError());
   return HEU_OK;
@@ -497,15 +538,44 @@ int heu_ctx_destroy(heu_ctx* ctx) {
   if (ctx->d_err) (void)hipFree(ctx->d_err);
   ctx->scratch.alloc(0);
   ctx->meta.alloc(0);
+  ctx->prof_clear();
+  for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
+  return HEU_OK;
+}
+
+int heu_ctx_profile(heu_ctx* ctx, int enable) {
+  if (!ctx) return fail(HEU_EINVAL, "null ctx");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->prof_clear();
+  ctx->prof = enable != 0;
+  return HEU_OK;
+}
+
+int heu_ctx_profile_read(heu_ctx* ctx, const char* name, double* total_ms, uint64_t* launches) {
+  if (!ctx || !name || !total_ms || !launches) return fail(HEU_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  double t = 0;
+  uint64_t n = 0;
+  for (auto& r : ctx->prof_recs) {
+    if (strcmp(r.name, name) != 0) continue;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, r.start, r.stop));
+    t += ms;
+    ++n;
+  }
+  *total_ms = t;
+  *launches = n;
   return HEU_OK;
 }
 
 int heu_ctx_set_stream(heu_ctx* ctx, void* stream) {
   if (!ctx) return fail(HEU_EINVAL, "null ctx");
   std::lock_guard<std::mutex> lk(ctx->mu);
-  ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+  ctx->stream = static_cast<hipStream_t>(stream);
   return HEU_OK;
 }
 
@@ -582,8 +652,8 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
     if ((rc = key->table.alloc(tbytes))) return rc;
     key->device_bytes += tbytes;
     for (int level = 0; level <= (int)window_bits; ++level)
-      ops->table_level(st, key->mod_n2(), dbases.as<uint32_t>(), key->table.as<uint32_t>(), key->windows,
-                       key->unit, level);
+      LAUNCH(ctx, "table_level", ops->table_level(st, key->mod_n2(), dbases.as<uint32_t>(), key->table.as<uint32_t>(), key->windows,
+                       key->unit, level));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
   }
@@ -683,7 +753,7 @@ int heu_encrypt_i64(heu_key* key, const int64_t* m, const uint32_t* r, uint32_t 
   if (count == 0) return HEU_OK;
   if (!m || !r || !ct || r_words == 0) return fail(HEU_EINVAL, "null argument");
   // |m| <= n/2 always holds for 64-bit m with n >= 2^66
-  o->encrypt_i64(ctx->stream, key->pub(), m, r, (int)r_words, (long)count, ct, ctx->d_err);
+  LAUNCH(ctx, "encrypt_i64", o->encrypt_i64(ctx->stream, key->pub(), m, r, (int)r_words, (long)count, ct, ctx->d_err));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }
@@ -716,7 +786,7 @@ int heu_encrypt(heu_key* key, const uint32_t* m, const uint32_t* m_host, uint32_
     if (BN_cmp(v, half) > 0)
       return fail(HEU_ERANGE, "message number out of range at index %zu, max (abs) = n/2", i);
   }
-  o->encrypt_big(ctx->stream, key->pub(), m, (int)m_words, r, (int)r_words, (long)count, ct, ctx->d_err);
+  LAUNCH(ctx, "encrypt_big", o->encrypt_big(ctx->stream, key->pub(), m, (int)m_words, r, (int)r_words, (long)count, ct, ctx->d_err));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }
@@ -726,7 +796,7 @@ int heu_randomize(heu_key* key, const uint32_t* ct, const uint32_t* r, uint32_t 
   KEY_PROLOGUE(key);
   if (count == 0) return HEU_OK;
   if (!r || !out || r_words == 0) return fail(HEU_EINVAL, "null argument");
-  o->randomize(ctx->stream, key->pub(), ct, r, (int)r_words, (long)count, out, ctx->d_err);
+  LAUNCH(ctx, "randomize", o->randomize(ctx->stream, key->pub(), ct, r, (int)r_words, (long)count, out, ctx->d_err));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }
@@ -747,9 +817,9 @@ int heu_decrypt(heu_key* key, const uint32_t* ct, size_t count, uint32_t* out, u
   const CombArgs C = key->comb();
   for (long s0 = 0; s0 < (long)count; s0 += slice) {
     const long n = std::min<long>(slice, (long)count - s0);
-    o->dec_half(ctx->stream, D, ct + s0 * key->ct_words, n, sc.at<uint32_t>(halves), sc.at<uint32_t>(tab));
-    o->combine(ctx->stream, C, sc.at<uint32_t>(halves), n, out + s0 * out_words, (int)out_words,
-               overflow ? overflow + s0 : nullptr);
+    LAUNCH(ctx, "dec_half", o->dec_half(ctx->stream, D, ct + s0 * key->ct_words, n, sc.at<uint32_t>(halves), sc.at<uint32_t>(tab)));
+    LAUNCH(ctx, "dec_combine", o->combine(ctx->stream, C, sc.at<uint32_t>(halves), n, out + s0 * out_words, (int)out_words,
+               overflow ? overflow + s0 : nullptr));
   }
   HIP_TRY(hipGetLastError());
   return HEU_OK;
@@ -759,7 +829,7 @@ int heu_add(heu_key* key, const uint32_t* a, const uint32_t* b, size_t count, ui
   KEY_PROLOGUE(key);
   if (count == 0) return HEU_OK;
   if (!a || !b || !out) return fail(HEU_EINVAL, "null argument");
-  o->mulmod(ctx->stream, key->mod_n2(), a, b, (long)count, out);
+  LAUNCH(ctx, "mulmod", o->mulmod(ctx->stream, key->mod_n2(), a, b, (long)count, out));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }
@@ -768,7 +838,7 @@ int heu_add_i64(heu_key* key, const uint32_t* ct, const int64_t* pt, size_t coun
   KEY_PROLOGUE(key);
   if (count == 0) return HEU_OK;
   if (!ct || !pt || !out) return fail(HEU_EINVAL, "null argument");
-  o->add_pt_i64(ctx->stream, key->pub(), ct, pt, (long)count, out);
+  LAUNCH(ctx, "add_pt_i64", o->add_pt_i64(ctx->stream, key->pub(), ct, pt, (long)count, out));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }
@@ -794,7 +864,7 @@ int heu_sub(heu_key* key, const uint32_t* a, const uint32_t* b, size_t count, ui
   int rc = sc.commit();
   if (rc) return rc;
   if ((rc = batch_invert(key, b, nullptr, (long)count, sc.at<uint32_t>(binv), base))) return rc;
-  o->mulmod(ctx->stream, key->mod_n2(), a, sc.at<uint32_t>(binv), (long)count, out);
+  LAUNCH(ctx, "mulmod", o->mulmod(ctx->stream, key->mod_n2(), a, sc.at<uint32_t>(binv), (long)count, out));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }
@@ -813,8 +883,8 @@ int heu_mul_i64(heu_key* key, const uint32_t* ct, const int64_t* k, size_t count
   if (rc) return rc;
   for (long s0 = 0; s0 < (long)count; s0 += slice) {
     const long n = std::min<long>(slice, (long)count - s0);
-    o->pow_i64(ctx->stream, key->mod_n2(), ct + s0 * key->ct_words, k + s0, n, out + s0 * key->ct_words,
-               sc.at<uint8_t>(neg) + s0, sc.at<uint32_t>(tab));
+    LAUNCH(ctx, "pow_i64", o->pow_i64(ctx->stream, key->mod_n2(), ct + s0 * key->ct_words, k + s0, n, out + s0 * key->ct_words,
+               sc.at<uint8_t>(neg) + s0, sc.at<uint32_t>(tab)));
   }
   HIP_TRY(hipGetLastError());
   // negative scalars: invert c^|k| (evaluator.cc:119, PowMod with a negative exponent)
@@ -853,7 +923,7 @@ static int mont_convert(heu_key* key, const uint32_t* ct, size_t count, uint32_t
   if (rc) return rc;
   auto words = bn_to_words(R, key->ct_words);
   if ((rc = upload_meta(ctx, sc, cz, words.data(), words.size() * 4))) return rc;
-  o->mul_const(ctx->stream, key->mod_n2(), ct, sc.at<uint32_t>(cz), (long)count, out);
+  LAUNCH(ctx, "mul_const", o->mul_const(ctx->stream, key->mod_n2(), ct, sc.at<uint32_t>(cz), (long)count, out));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }

@@ -125,6 +125,16 @@ class Context:
     def sync(self) -> None:
         check(lib().heu_ctx_sync(self._h))
 
+    def profile(self, enable: bool = True) -> None:
+        """Per-kernel event timing on/off (clears the records)."""
+        check(lib().heu_ctx_profile(self._h, int(enable)))
+
+    def profile_read(self, kernel: str):
+        """(total milliseconds, launches) of `kernel` since profiling was enabled."""
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        check(lib().heu_ctx_profile_read(self._h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and _native._lib is not None:
             lib().heu_ctx_destroy(self._h)

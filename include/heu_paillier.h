@@ -51,13 +51,21 @@ const char *heu_last_error(void);
 /* Creates a context on HIP device `device` with its own stream. */
 int heu_ctx_create(int device, heu_ctx **out);
 int heu_ctx_destroy(heu_ctx *ctx);
-/* Makes subsequent work run on `stream` (a hipStream_t), e.g. the caller's
- * framework stream.  NULL restores the context's own stream. */
+/* Makes subsequent work run on `stream` (a hipStream_t, used as given: NULL is
+ * the device's null stream), e.g. the caller's framework stream.  A new
+ * context uses a stream of its own. */
 int heu_ctx_set_stream(heu_ctx *ctx, void *stream);
 void *heu_ctx_stream(heu_ctx *ctx);
 /* Waits for all enqueued work; returns HEU_ERANGE / HEU_ENOTINV if a kernel
  * flagged bad input since the last sync, then clears the flags. */
 int heu_ctx_sync(heu_ctx *ctx);
+
+/* Per-kernel timing (tracing): with enable != 0, every kernel launch of this
+ * context is bracketed by HIP events on its stream; enabling or disabling
+ * clears the records.  heu_ctx_profile_read sums the elapsed time of the
+ * launches of kernel `name` ("encrypt_i64", "dec_half", "pow_i64", ...). */
+int heu_ctx_profile(heu_ctx *ctx, int enable);
+int heu_ctx_profile_read(heu_ctx *ctx, const char *name, double *total_ms, uint64_t *launches);
 
 /* ---- key generation (paillier_zahlen/key_generator.cc:26-60) ------------ */
 /* Writes n (key_bits/32 limbs), h_s (key_bits/16), p and q (key_bits/64 each).
