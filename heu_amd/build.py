@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libheu_paillier.so")
 ARCH = os.environ.get("HEU_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["paillier_kernels.hip", "engine.cpp", "keygen.cpp"]
+SOURCES = ["paillier_kernels.hip", "engine.cpp", "keygen.cpp"]  # valu_peak.hip builds separately
 # Packed FP32 instructions stay off in device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -55,6 +55,14 @@ def build(force: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # the VALU issue-rate probe (a standalone executable)
+    probe = os.path.join(HERE, "valu_peak")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", *DEVICE_FLAGS, "-o", probe + ".tmp",
+           os.path.join(CSRC, "valu_peak.hip")]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{res.stderr}")
+    os.replace(probe + ".tmp", probe)
     return LIB
 
 
