@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -597,7 +598,7 @@ int heu_ctx_sync(heu_ctx* ctx) {
 int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint32_t n_words, const uint32_t* p_w,
                    const uint32_t* q_w, uint32_t prime_words, uint32_t r_bits, uint32_t window_bits, heu_key** out) {
   if (!ctx || !n_w || !hs_w || !out) return fail(HEU_EINVAL, "null argument");
-  if (window_bits < 1 || window_bits > 16) return fail(HEU_EINVAL, "window_bits must be in [1, 16]");
+  if (window_bits > 16) return fail(HEU_EINVAL, "window_bits must be in [0, 16] (0 = automatic)");
   if (r_bits < 1 || r_bits > 32 * n_words) return fail(HEU_EINVAL, "r_bits must be in [1, bits(n)]");
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
@@ -633,7 +634,20 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
     BN_rshift1(h, n);
     key->nhalf_host = bn_to_words(h, 2 * ops->p_words);
   }
-  // fixed-base table
+  // fixed-base table; window 0 = the widest window whose table fits the
+  // per-key budget (HEU_TABLE_BYTES, default 1 GiB), at least the reference's 10
+  if (window_bits == 0) {
+    size_t budget = (size_t)1 << 30;
+    if (const char* b = getenv("HEU_TABLE_BYTES")) budget = (size_t)strtoull(b, nullptr, 10);
+    window_bits = 10;
+    for (uint32_t w = 16; w > 10; --w) {
+      const size_t bytes = ((size_t)((r_bits + w - 1) / w) << w) * (size_t)key->nl_n * 4;
+      if (bytes <= budget) {
+        window_bits = w;
+        break;
+      }
+    }
+  }
   key->unit = (int)window_bits;
   key->windows = (int)((r_bits + window_bits - 1) / window_bits);
   {

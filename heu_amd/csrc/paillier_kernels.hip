@@ -875,15 +875,111 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(Mod
   }
 }
 
-// Inverse of one Montgomery-form value (R29), in place: to Z-space, binary
-// extended Euclid on 32-bit words by one thread, back to Montgomery form.
-// Sets kErrNotInvertible if gcd(x, m) != 1.
+// ---------------------------------------------------------------------------
+// The one inversion per batch, by binary extended Euclid on wave-wide big
+// integers: a W-word number is spread over the 64 lanes of one wave,
+// WPL = W/64 consecutive words per lane.  Carries and borrows cross lanes by
+// carry lookahead on ballot masks: with G = lanes that generate a carry and
+// P = lanes that pass an incoming one on, bit i of ((G|P) + G) ^ (G|P) ^ G is
+// the carry into lane i, and the 64-bit addition's overflow is the carry out.
+// ---------------------------------------------------------------------------
+template <int WPL>
+struct WaveNum {
+  u32 w[WPL];
+};
+
+// a += b (SUB: a -= b) over the whole wave; returns the carry (borrow) out.
+template <int WPL, bool SUB>
+__device__ __forceinline__ bool wn_addsub(WaveNum<WPL>& a, const WaveNum<WPL>& b, u32 lane) {
+  bool gen, prop = true;
+  if constexpr (!SUB) {
+    u64 c = 0;
+#pragma unroll
+    for (int j = 0; j < WPL; ++j) {
+      u64 s = (u64)a.w[j] + b.w[j] + c;
+      a.w[j] = (u32)s;
+      c = s >> 32;
+      prop &= a.w[j] == 0xffffffffu;
+    }
+    gen = c != 0;
+  } else {
+    i64 br = 0;
+#pragma unroll
+    for (int j = 0; j < WPL; ++j) {
+      i64 s = (i64)a.w[j] - (i64)b.w[j] + br;
+      a.w[j] = (u32)s;
+      br = s >> 32;
+      prop &= a.w[j] == 0u;
+    }
+    gen = br != 0;
+  }
+  const u64 G = __builtin_amdgcn_ballot_w64(gen), P = __builtin_amdgcn_ballot_w64(prop), X = G | P;
+  const u64 S = X + G;
+  if (((S ^ X ^ G) >> lane) & 1u) {
+    bool go = true;
+#pragma unroll
+    for (int j = 0; j < WPL; ++j) {
+      if (go) {
+        if constexpr (!SUB) {
+          a.w[j] += 1u;
+          go = a.w[j] == 0u;
+        } else {
+          a.w[j] -= 1u;
+          go = a.w[j] == 0xffffffffu;
+        }
+      }
+    }
+  }
+  return S < X;
+}
+
+// a >>= 1, with `top` shifted in as the new most significant bit.
+template <int WPL>
+__device__ __forceinline__ void wn_shr1(WaveNum<WPL>& a, u32 lane, u32 top) {
+  u32 nxt = (u32)__shfl_down((int)a.w[0], 1, 64);
+  if (lane == 63u) nxt = top;
+#pragma unroll
+  for (int j = 0; j < WPL; ++j) a.w[j] = (a.w[j] >> 1) | ((j + 1 < WPL ? a.w[j + 1] : nxt) << 31);
+}
+
+template <int WPL>
+__device__ __forceinline__ bool wn_is_small(const WaveNum<WPL>& a, u32 lane, u32 value) {
+  bool other = false;
+#pragma unroll
+  for (int j = 0; j < WPL; ++j) other |= a.w[j] != ((lane == 0u && j == 0) ? value : 0u);
+  return __builtin_amdgcn_ballot_w64(other) == 0;
+}
+
+template <int WPL>
+__device__ __forceinline__ bool wn_even(const WaveNum<WPL>& a) {
+  return (__builtin_amdgcn_readfirstlane(a.w[0]) & 1u) == 0u;  // lane 0 holds the low word
+}
+
+// a/2 mod m for a < m, m odd
+template <int WPL>
+__device__ __forceinline__ void wn_halve_mod(WaveNum<WPL>& a, const WaveNum<WPL>& m, u32 lane) {
+  u32 top = 0;
+  if (!wn_even(a)) top = wn_addsub<WPL, false>(a, m, lane) ? 1u : 0u;
+  wn_shr1(a, lane, top);
+}
+
+// a - b mod m for a, b < m
+template <int WPL>
+__device__ __forceinline__ void wn_sub_mod(WaveNum<WPL>& a, const WaveNum<WPL>& b, const WaveNum<WPL>& m, u32 lane) {
+  if (wn_addsub<WPL, true>(a, b, lane)) wn_addsub<WPL, false>(a, m, lane);
+}
+
+// Inverse of one Montgomery-form value (R29), in place: a group of L lanes maps
+// it to canonical Z-space words, the whole wave runs the extended Euclid, the
+// group maps the result back (x R -> x^-1 R).  Sets kErrNotInvertible if
+// gcd(x, m) != 1.  One wave, launched once per batch.
 template <int L, int K, int W>
 __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __restrict__ m_words, u32* xio, u32* err) {
   constexpr int NL = L * K;
+  constexpr int WPL = W / 64;
+  static_assert(W % 64 == 0, "wave-wide numbers need W divisible by 64");
   __shared__ u32 lds[NL + 2 * Cfg<L, K, W>::S];
-  __shared__ u32 u[W + 1], v[W + 1], x1[W + 1], x2[W + 1], mm[W + 1];
-  __shared__ u32 fail;
+  __shared__ u32 words[W];
   for (int i = threadIdx.x; i < NL; i += 64) lds[i] = mod.m[i];
   __syncthreads();
   Grp<L, K, W> g;
@@ -900,103 +996,54 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
     get_lds<L, K>(x, g.A, g.lid);
     canonicalize<L, K>(x, g.md);
     g.set(g.A, x);
-    for (int w = (int)g.lid; w < W; w += L) u[w] = limbs_to_word<L, K>(g.A, w);
+    for (int w = (int)g.lid; w < W; w += L) words[w] = limbs_to_word<L, K>(g.A, w);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < W; ++i) {
-      v[i] = m_words[i];
-      mm[i] = m_words[i];
-      x1[i] = i == 0 ? 1u : 0u;
-      x2[i] = 0u;
-    }
-    auto is_one = [&](const u32* a) {
-      if (a[0] != 1u) return false;
-      for (int i = 1; i < W; ++i)
-        if (a[i]) return false;
-      return true;
-    };
-    auto is_zero = [&](const u32* a) {
-      for (int i = 0; i < W; ++i)
-        if (a[i]) return false;
-      return true;
-    };
-    auto halve = [&](u32* a) {
-      for (int i = 0; i < W; ++i) a[i] = (a[i] >> 1) | (i + 1 < W ? (a[i + 1] << 31) : 0u);
-    };
-    auto halve_mod = [&](u32* a) {  // a/2 mod m (a < m, m odd)
-      u64 c = 0;
-      if (a[0] & 1u) {
-        for (int i = 0; i < W; ++i) {
-          u64 s = (u64)a[i] + mm[i] + c;
-          a[i] = (u32)s;
-          c = s >> 32;
-        }
-      }
-      for (int i = 0; i < W; ++i) a[i] = (a[i] >> 1) | (i + 1 < W ? (a[i + 1] << 31) : ((u32)c << 31));
-    };
-    auto geq = [&](const u32* a, const u32* bb) {
-      for (int i = W - 1; i >= 0; --i)
-        if (a[i] != bb[i]) return a[i] > bb[i];
-      return true;
-    };
-    auto sub = [&](u32* a, const u32* bb) {
-      i64 br = 0;
-      for (int i = 0; i < W; ++i) {
-        i64 s = (i64)a[i] - bb[i] + br;
-        a[i] = (u32)s;
-        br = s >> 32;
-      }
-    };
-    auto sub_mod = [&](u32* a, const u32* bb) {  // a - b mod m for a, b < m
-      i64 br = 0;
-      for (int i = 0; i < W; ++i) {
-        i64 s = (i64)a[i] - bb[i] + br;
-        a[i] = (u32)s;
-        br = s >> 32;
-      }
-      if (br < 0) {
-        u64 c = 0;
-        for (int i = 0; i < W; ++i) {
-          u64 s = (u64)a[i] + mm[i] + c;
-          a[i] = (u32)s;
-          c = s >> 32;
-        }
-      }
-    };
-    u32 f = is_zero(u) ? 1u : 0u;
-    long guard = 0;
-    const long kMaxSteps = 4l * 32 * W + 64;
-    while (!f && !is_one(u) && !is_one(v)) {
-      while (!(u[0] & 1u)) {
-        halve(u);
-        halve_mod(x1);
-      }
-      while (!(v[0] & 1u)) {
-        halve(v);
-        halve_mod(x2);
-      }
-      if (geq(u, v)) {
-        sub(u, v);
-        sub_mod(x1, x2);
-      } else {
-        sub(v, u);
-        sub_mod(x2, x1);
-      }
-      if (is_zero(u) || is_zero(v) || ++guard > kMaxSteps) f = 1u;
-    }
-    if (!f && !is_one(u))
-      for (int i = 0; i < W; ++i) x1[i] = x2[i];
-    fail = f;
+  const u32 lane = threadIdx.x;
+  WaveNum<WPL> u, v, x1, x2, m;
+#pragma unroll
+  for (int j = 0; j < WPL; ++j) {
+    u.w[j] = words[lane * WPL + j];
+    m.w[j] = m_words[lane * WPL + j];
+    v.w[j] = m.w[j];
+    x1.w[j] = (lane == 0u && j == 0) ? 1u : 0u;
+    x2.w[j] = 0u;
   }
-  __syncthreads();
+  // invariant: x1 * x = u, x2 * x = v (mod m)
+  bool fail = wn_is_small(u, lane, 0u);
+  const int max_steps = 4 * 32 * W + 64;
+  int steps = 0;
+  while (!fail && !wn_is_small(u, lane, 1u) && !wn_is_small(v, lane, 1u)) {
+    while (wn_even(u)) {
+      wn_shr1(u, lane, 0u);
+      wn_halve_mod(x1, m, lane);
+    }
+    while (wn_even(v)) {
+      wn_shr1(v, lane, 0u);
+      wn_halve_mod(x2, m, lane);
+    }
+    WaveNum<WPL> d = u;
+    if (!wn_addsub<WPL, true>(d, v, lane)) {  // u >= v
+      u = d;
+      wn_sub_mod(x1, x2, m, lane);
+    } else {
+      wn_addsub<WPL, true>(v, u, lane);
+      wn_sub_mod(x2, x1, m, lane);
+    }
+    if (wn_is_small(u, lane, 0u) || wn_is_small(v, lane, 0u) || ++steps > max_steps) fail = true;
+  }
   if (fail) {
-    if (threadIdx.x == 0) atomicOr(err, kErrNotInvertible);
-    return;
+    if (lane == 0u) atomicOr(err, kErrNotInvertible);
+    return;  // uniform: every lane saw the same ballots
   }
+  const bool u_one = wn_is_small(u, lane, 1u);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < WPL; ++j) words[lane * WPL + j] = u_one ? x1.w[j] : x2.w[j];
+  __syncthreads();
   if (in_grp) {
     u32 x[K];
-    words_to_limbs<L, K>(x1, W, 0, x);
+    words_to_limbs<L, K>(words, W, 0, x);
     g.set(g.A, x);
     g.mul_global(mod.r2);
     g.save(xio, g.A);

@@ -163,7 +163,7 @@ class Key:
     """A Paillier key resident on the device (public part, optional secret part)."""
 
     def __init__(self, ctx: Context, n: int, hs: int, p: int | None = None, q: int | None = None,
-                 r_bits: int = 256, window_bits: int = 10):
+                 r_bits: int = 256, window_bits: int = 0):
         self.ctx = ctx
         self.n, self.hs, self.p, self.q = n, hs, p, q
         n_words = (n.bit_length() + 31) // 32
