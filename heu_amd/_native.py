@@ -39,6 +39,7 @@ SIGNATURES = {
     "heu_sub": (I, [P, P, P, SZ, P]),
     "heu_negate": (I, [P, P, SZ, P]),
     "heu_mul_i64": (I, [P, P, P, SZ, P]),
+    "heu_mul": (I, [P, P, P, P, U32, SZ, P]),
     "heu_sum": (I, [P, P, P, SZ, P]),
     "heu_dot_i64": (I, [P, P, P, P, SZ, P]),
     "heu_to_mont": (I, [P, P, SZ, U32, P]),

@@ -63,7 +63,38 @@ def build(force: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{res.stderr}")
     os.replace(probe + ".tmp", probe)
+    build_spi()
     return LIB
+
+
+SPI_LIB = os.path.join(HERE, "libheu_paillier_spi.so")
+SPI_TEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "build", "paillier_hip_test")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def build_spi() -> str:
+    """The C++ vectorized-SPI classes (host code over the C ABI) and their test
+    binary; both host-only, so g++ builds them."""
+    include = os.path.join(os.path.dirname(HERE), "include")
+    spi = os.path.join(HERE, "spi")
+    common = ["g++", "-std=c++20", "-O2", "-fPIC", "-Wall", f"-I{include}", f"-I{spi}", f"-I{ROCM}/include",
+              "-D__HIP_PLATFORM_AMD__"]
+    cmd = [*common, "-shared", "-o", SPI_LIB + ".tmp", os.path.join(spi, "paillier_hip.cc"), f"-L{HERE}",
+           "-lheu_paillier", f"-L{ROCM}/lib", "-lamdhip64", "-lcrypto", "-Wl,-rpath,$ORIGIN",
+           f"-Wl,-rpath,{ROCM}/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"SPI build failed: {' '.join(cmd)}\n{res.stderr}")
+    os.replace(SPI_LIB + ".tmp", SPI_LIB)
+    os.makedirs(os.path.dirname(SPI_TEST), exist_ok=True)
+    src = os.path.join(os.path.dirname(os.path.dirname(SPI_TEST)), "paillier_hip_test.cc")
+    cmd = [*common, "-o", SPI_TEST + ".tmp", src, f"-L{HERE}", "-lheu_paillier_spi", "-lheu_paillier",
+           f"-L{ROCM}/lib", "-lamdhip64", "-lcrypto", "-Wl,-rpath,$ORIGIN/../../../heu_amd", f"-Wl,-rpath,{ROCM}/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"SPI test build failed: {' '.join(cmd)}\n{res.stderr}")
+    os.replace(SPI_TEST + ".tmp", SPI_TEST)
+    return SPI_LIB
 
 
 if __name__ == "__main__":

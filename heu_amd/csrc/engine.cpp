@@ -905,6 +905,30 @@ int heu_mul_i64(heu_key* key, const uint32_t* ct, const int64_t* k, size_t count
   return batch_invert(key, out, sc.at<uint8_t>(neg), (long)count, out, base);
 }
 
+int heu_mul(heu_key* key, const uint32_t* ct, const uint32_t* k_mag, const uint8_t* k_neg, uint32_t k_words,
+            size_t count, uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (count == 0) return HEU_OK;
+  if (!ct || !k_mag || !out || k_words == 0) return fail(HEU_EINVAL, "null argument");
+  const long slice = std::min<long>((long)count, kSliceElems);
+  const size_t NLB = (size_t)key->nl_n * 4;
+  Scratch sc(ctx);
+  size_t tab = sc.add((size_t)slice * (NLB << heu_hip::kPowWindowBits));
+  size_t neg = sc.add(count);
+  size_t base = sc.add(inv_scratch_bytes(key, (long)count));
+  int rc = sc.commit();
+  if (rc) return rc;
+  for (long s0 = 0; s0 < (long)count; s0 += slice) {
+    const long n = std::min<long>(slice, (long)count - s0);
+    LAUNCH(ctx, "pow_words",
+           o->pow_words(ctx->stream, key->mod_n2(), ct + s0 * key->ct_words, k_mag + s0 * k_words,
+                        k_neg ? k_neg + s0 : nullptr, (int)k_words, n, out + s0 * key->ct_words,
+                        sc.at<uint8_t>(neg) + s0, sc.at<uint32_t>(tab)));
+  }
+  HIP_TRY(hipGetLastError());
+  return batch_invert(key, out, sc.at<uint8_t>(neg), (long)count, out, base);
+}
+
 int heu_sum(heu_key* key, const uint32_t* ct, const int64_t* offsets, size_t segments, uint32_t* out) {
   KEY_PROLOGUE(key);
   if (!offsets || !out || (!ct && segments && offsets[segments] > offsets[0])) return fail(HEU_EINVAL, "null argument");

@@ -62,6 +62,8 @@ typedef void (*RandFn)(hipStream_t, const PubArgs&, const uint32_t*, const uint3
 typedef void (*MulModFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
 typedef void (*AddPtFn)(hipStream_t, const PubArgs&, const uint32_t*, const int64_t*, long, uint32_t*);
 typedef void (*PowFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, uint32_t*, u8*, uint32_t*);
+typedef void (*PowWordsFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, const u8*, int, long,
+                           uint32_t*, u8*, uint32_t*);
 typedef void (*ReduceSumFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const Chunk*, long, uint32_t*);
 typedef void (*ReduceDot0Fn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, const Chunk*, long,
                              uint32_t*, uint32_t*, uint32_t*);
@@ -88,6 +90,7 @@ struct Ops {
   MulModFn mulmod;
   AddPtFn add_pt_i64;
   PowFn pow_i64;
+  PowWordsFn pow_words;
   ReduceSumFn reduce_sum;
   ReduceDot0Fn reduce_dot0;
   ReduceMontFn reduce_mont;

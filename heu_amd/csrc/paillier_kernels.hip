@@ -401,6 +401,57 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_pow_i64(ModA
   g.store_canonical(out + e * W, W);
 }
 
+// ct (x) scalar for exponents of any length: |k| as kw little-endian words
+// (kmag) and its sign (kneg, may be null).  Same special cases as k_pow_i64.
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_pow_words(ModArgs mod, const u32* __restrict__ ct,
+                                                                            const u32* __restrict__ kmag,
+                                                                            const u8* __restrict__ kneg, int kw,
+                                                                            long count, u32* __restrict__ out, u8* neg,
+                                                                            u32* tab) {
+  constexpr int NL = L * K;
+  constexpr u32 kMask = (1u << kPowWin) - 1u;
+  constexpr int kPerWord = 32 / kPowWin;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  const u32* kp = kmag + e * kw;
+  int topw = kw - 1;
+  while (topw >= 0 && kp[topw] == 0u) --topw;
+  const bool is_neg = kneg && kneg[e] && topw >= 0;
+  if (neg && g.lid == 0) neg[e] = is_neg ? 1 : 0;
+  if (topw < 0 || (topw == 0 && kp[0] == 1u && !is_neg)) {
+    for (int w = (int)g.lid; w < W; w += L) out[e * W + w] = topw < 0 ? (w == 0 ? 1u : 0u) : ct[e * W + w];
+    return;
+  }
+  u32 x[K];
+  g.load_words(x, ct + e * W, W);
+  g.set(g.A, x);
+  g.mul_global(mod.r2);  // c R
+  u32* T = tab + e * (long)(NL << kPowWin);
+  load_r29<L, K>(x, mod.one, g.lid);
+  store_r29<L, K>(T, x, g.lid);
+  get_lds<L, K>(x, g.A, g.lid);
+  store_r29<L, K>(T + NL, x, g.lid);
+  g.set(g.B, x);
+  for (int d = 2; d < (1 << kPowWin); ++d) {
+    g.mul_B();
+    g.save(T + d * NL, g.A);
+  }
+  auto digit = [&](int i) { return (kp[i / kPerWord] >> (kPowWin * (i % kPerWord))) & kMask; };
+  int top = (topw + 1) * kPerWord - 1;
+  while (top > 0 && digit(top) == 0u) --top;
+  g.set_global(g.A, T + digit(top) * NL);
+  for (int i = top - 1; i >= 0; --i) {
+#pragma unroll 1
+    for (int s = 0; s < kPowWin; ++s) g.square();
+    g.mul_global(T + digit(i) * NL);
+  }
+  g.from_mont();
+  g.store_canonical(out + e * W, W);
+}
+
 // ---------------------------------------------------------------------------
 // CRT decryption, half h of element e (decryptor.cc:32-53):
 //   u_h = c^(p_h - 1) mod p_h^2, canonical, written as W/2 words.
@@ -1118,6 +1169,10 @@ struct Launch {
                       u32* tab) {
     HEU_LAUNCH((k_pow_i64<L, K, W>), n, L, st, mod, ct, k, n, out, neg, tab);
   }
+  static void pow_words(hipStream_t st, const ModArgs& mod, const u32* ct, const u32* kmag, const u8* kneg, int kw,
+                        long n, u32* out, u8* neg, u32* tab) {
+    HEU_LAUNCH((k_pow_words<L, K, W>), n, L, st, mod, ct, kmag, kneg, kw, n, out, neg, tab);
+  }
   static void reduce_sum(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const Chunk* chunks, long nch,
                          u32* part) {
     if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part);
@@ -1194,6 +1249,7 @@ static const Ops make_ops() {
   o.mulmod = Launch<SN>::mulmod;
   o.add_pt_i64 = Launch<SN>::add_pt_i64;
   o.pow_i64 = Launch<SN>::pow_i64;
+  o.pow_words = Launch<SN>::pow_words;
   o.reduce_sum = Launch<SN>::reduce_sum;
   o.reduce_dot0 = Launch<SN>::reduce_dot0;
   o.reduce_mont = Launch<SN>::reduce_mont;

@@ -304,6 +304,18 @@ class Key:
         check(lib().heu_mul_i64(self._h, _ptr(a), _ptr(k), n, _ptr(out)))
         return out
 
+    def mul(self, a, scalars) -> torch.Tensor:
+        """ct x scalar for Python ints of any size (evaluator.cc:99-123)."""
+        a = self._dev(a, torch.int32)
+        n = self._rows(a)
+        mags = [abs(int(v)) for v in scalars]
+        kw = max(1, max((m.bit_length() + 31) // 32 for m in mags) if mags else 1)
+        kmag = self._dev(ints_to_words(mags, kw), torch.int32)
+        kneg = self._dev(np.array([int(v) < 0 for v in scalars], dtype=np.uint8), torch.uint8)
+        out = self._ct_out(n)
+        check(lib().heu_mul(self._h, _ptr(a), _ptr(kmag), _ptr(kneg), kw, n, _ptr(out)))
+        return out
+
     def sum(self, ct, offsets) -> torch.Tensor:
         ct = self._dev(ct, torch.int32)
         offs = np.ascontiguousarray(offsets, dtype=np.int64)

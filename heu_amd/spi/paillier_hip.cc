@@ -1,0 +1,914 @@
+// paillier_hip.cc -- the vectorized-SPI classes of paillier_hip.h over the
+// engine's C ABI (include/heu_paillier.h).
+//
+// Every batch call packs its operands into flat limb arrays, stages them in a
+// device arena on the engine context's stream, runs one engine entry point and
+// copies the results back.  Calls from several threads serialize on the
+// process-wide runtime lock (one context, one stream).
+#include "paillier_hip.h"
+
+#include <hip/hip_runtime_api.h>
+#include <openssl/bn.h>
+#include <openssl/crypto.h>
+#include <openssl/rand.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <stdexcept>
+
+#include "heu_paillier.h"
+
+namespace heu_amd {
+
+// ------------------------------------------------------------------ BigInt
+namespace {
+
+BN_CTX* tls_ctx() {
+  thread_local std::unique_ptr<BN_CTX, void (*)(BN_CTX*)> c(BN_CTX_new(), BN_CTX_free);
+  return c.get();
+}
+
+[[noreturn]] void throw_msg(const std::string& m) { throw std::runtime_error(m); }
+
+void bn_check(int ok, const char* what) {
+  if (!ok) throw_msg(std::string("BIGNUM ") + what + " failed");
+}
+
+}  // namespace
+
+BigInt::BigInt() : b_(BN_new()) {
+  if (!b_) throw std::bad_alloc();
+}
+
+BigInt::BigInt(int64_t v) : BigInt() {
+  uint64_t mag = v < 0 ? 0ull - (uint64_t)v : (uint64_t)v;
+  bn_check(BN_set_word(b_, mag), "set_word");
+  BN_set_negative(b_, v < 0);
+}
+
+BigInt::BigInt(const BigInt& o) : b_(BN_dup(o.b_)) {
+  if (!b_) throw std::bad_alloc();
+}
+
+BigInt::BigInt(BigInt&& o) noexcept : b_(o.b_) { o.b_ = nullptr; }
+
+BigInt& BigInt::operator=(const BigInt& o) {
+  if (this != &o) {
+    if (!b_) b_ = BN_new();
+    bn_check(BN_copy(b_, o.b_) != nullptr, "copy");
+  }
+  return *this;
+}
+
+BigInt& BigInt::operator=(BigInt&& o) noexcept {
+  std::swap(b_, o.b_);
+  return *this;
+}
+
+BigInt::~BigInt() { BN_free(b_); }
+
+BigInt BigInt::FromString(const std::string& s, int base) {
+  BigInt r;
+  BIGNUM* p = r.b_;
+  int n = base == 16 ? BN_hex2bn(&p, s.c_str()) : BN_dec2bn(&p, s.c_str());
+  if (n == 0 || (size_t)n + (s[0] == '-') < s.size()) throw_msg("BigInt: cannot parse '" + s + "'");
+  return r;
+}
+
+BigInt BigInt::FromWords(const uint32_t* w, size_t words, bool twos_complement) {
+  BigInt r;
+  if (words == 0) return r;
+  bn_check(BN_lebin2bn(reinterpret_cast<const unsigned char*>(w), (int)(words * 4), r.b_) != nullptr, "lebin2bn");
+  if (twos_complement && (w[words - 1] >> 31)) {
+    BigInt mod;
+    bn_check(BN_lshift(mod.b_, BN_value_one(), (int)(32 * words)), "lshift");
+    bn_check(BN_sub(r.b_, r.b_, mod.b_), "sub");
+  }
+  return r;
+}
+
+void BigInt::ToWords(uint32_t* w, size_t words) const {
+  // fits iff -2^(cap-1) <= v < 2^(cap-1), cap = 32 words
+  const size_t cap = 32 * words, bits = BitCount();
+  bool fits = bits < cap;
+  if (!fits && IsNegative() && bits == cap) {  // only v = -2^(cap-1) has a cap-bit magnitude and fits
+    BigInt t(*this);
+    BN_clear_bit(t.b_, (int)cap - 1);
+    fits = t.IsZero();
+  }
+  if (!fits) throw_msg("BigInt: value does not fit " + std::to_string(words) + " words");
+  if (!IsNegative()) {
+    bn_check(BN_bn2lebinpad(b_, reinterpret_cast<unsigned char*>(w), (int)(words * 4)) >= 0, "bn2lebinpad");
+    return;
+  }
+  BigInt t;
+  bn_check(BN_lshift(t.b_, BN_value_one(), (int)(32 * words)), "lshift");
+  bn_check(BN_add(t.b_, t.b_, b_), "add");  // 2^(32 words) - |v|
+  bn_check(BN_bn2lebinpad(t.b_, reinterpret_cast<unsigned char*>(w), (int)(words * 4)) >= 0, "bn2lebinpad");
+}
+
+std::string BigInt::ToString() const {
+  char* s = BN_bn2dec(b_);
+  std::string out(s);
+  OPENSSL_free(s);
+  return out;
+}
+
+std::string BigInt::ToHexString() const {
+  char* s = BN_bn2hex(b_);
+  std::string out(s);
+  OPENSSL_free(s);
+  return out;
+}
+
+size_t BigInt::BitCount() const { return (size_t)BN_num_bits(b_); }
+bool BigInt::IsNegative() const { return BN_is_negative(b_) != 0; }
+bool BigInt::IsZero() const { return BN_is_zero(b_) != 0; }
+int BigInt::Compare(const BigInt& o) const { return BN_cmp(b_, o.b_); }
+int BigInt::CompareAbs(const BigInt& o) const { return BN_ucmp(b_, o.b_); }
+
+bool BigInt::FitsInt64() const {
+  size_t bits = BitCount();
+  if (bits <= 63) return true;
+  // INT64_MIN = -2^63
+  return bits == 64 && IsNegative() && BN_get_word(b_) == (1ull << 63);
+}
+
+int64_t BigInt::ToInt64() const {
+  if (!FitsInt64()) throw_msg("BigInt: value does not fit int64");
+  uint64_t mag = BN_get_word(b_);
+  return IsNegative() ? (int64_t)(0ull - mag) : (int64_t)mag;
+}
+
+BigInt BigInt::operator+(const BigInt& o) const {
+  BigInt r;
+  bn_check(BN_add(r.b_, b_, o.b_), "add");
+  return r;
+}
+
+BigInt BigInt::operator-(const BigInt& o) const {
+  BigInt r;
+  bn_check(BN_sub(r.b_, b_, o.b_), "sub");
+  return r;
+}
+
+BigInt BigInt::operator*(const BigInt& o) const {
+  BigInt r;
+  bn_check(BN_mul(r.b_, b_, o.b_, tls_ctx()), "mul");
+  return r;
+}
+
+BigInt BigInt::operator-() const {
+  BigInt r(*this);
+  if (!r.IsZero()) BN_set_negative(r.b_, !IsNegative());
+  return r;
+}
+
+BigInt BigInt::operator>>(int bits) const {
+  BigInt r;
+  bn_check(BN_rshift(r.b_, b_, bits), "rshift");
+  return r;
+}
+
+namespace paillier_hip {
+
+// ------------------------------------------------------------------ runtime
+namespace {
+
+void ck(int rc) {
+  if (rc != HEU_OK) {
+    const char* m = heu_last_error();
+    throw_msg(std::string("heu engine error ") + std::to_string(rc) + ": " + (m ? m : ""));
+  }
+}
+
+void hip_ck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw_msg(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// One engine context per process (device from HEU_DEVICE, default 0) and a
+// growable device arena for staging batch operands.  Deliberately never
+// destroyed: HIP's own teardown at exit must not race a late hipFree.
+struct Runtime {
+  heu_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  char* arena = nullptr;
+  size_t cap = 0;
+
+  static Runtime& get() {
+    static Runtime* rt = [] {
+      auto* r = new Runtime;
+      const char* d = std::getenv("HEU_DEVICE");
+      ck(heu_ctx_create(d ? std::atoi(d) : 0, &r->ctx));
+      r->stream = static_cast<hipStream_t>(heu_ctx_stream(r->ctx));
+      return r;
+    }();
+    return *rt;
+  }
+
+  char* reserve(size_t bytes) {
+    if (bytes > cap) {
+      hip_ck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      if (arena) hip_ck(hipFree(arena), "hipFree");
+      arena = nullptr;
+      cap = 0;
+      size_t want = std::max(bytes, cap * 2);
+      hip_ck(hipMalloc(reinterpret_cast<void**>(&arena), want), "hipMalloc");
+      cap = want;
+    }
+    return arena;
+  }
+};
+
+// Stages one call's operands: inputs are copied up, outputs copied back after
+// the engine call, then the stream is synchronized and device flags checked.
+class DeviceCall {
+ public:
+  DeviceCall() : rt_(Runtime::get()), lock_(rt_.mu) {}
+
+  size_t in(const void* host, size_t bytes) {
+    size_t h = slot(bytes);
+    ups_.push_back({host, bytes, h});
+    return h;
+  }
+  size_t out(size_t bytes) { return slot(bytes); }
+
+  template <class T>
+  T* ptr(size_t h) {
+    return reinterpret_cast<T*>(base_ + offs_[h]);
+  }
+
+  void upload() {
+    base_ = rt_.reserve(total_ ? total_ : 256);
+    for (auto& u : ups_)
+      if (u.bytes)
+        hip_ck(hipMemcpyAsync(base_ + offs_[u.h], u.host, u.bytes, hipMemcpyHostToDevice, rt_.stream),
+               "hipMemcpyAsync H2D");
+  }
+
+  void download(size_t h, void* host, size_t bytes) {
+    if (bytes)
+      hip_ck(hipMemcpyAsync(host, base_ + offs_[h], bytes, hipMemcpyDeviceToHost, rt_.stream),
+             "hipMemcpyAsync D2H");
+  }
+
+  void finish() { ck(heu_ctx_sync(rt_.ctx)); }
+
+ private:
+  struct Up {
+    const void* host;
+    size_t bytes;
+    size_t h;
+  };
+  size_t slot(size_t bytes) {
+    offs_.push_back(total_);
+    total_ += (bytes + 255) & ~size_t(255);
+    return offs_.size() - 1;
+  }
+
+  Runtime& rt_;
+  std::lock_guard<std::mutex> lock_;
+  std::vector<size_t> offs_;
+  std::vector<Up> ups_;
+  size_t total_ = 0;
+  char* base_ = nullptr;
+};
+
+}  // namespace
+
+struct DeviceKey {
+  heu_key* key = nullptr;
+  uint32_t key_bits = 0, n_words = 0, ct_words = 0, r_words = 0;
+  bool has_sk = false;
+  std::vector<uint32_t> n2;  // n^2 limbs, for the ciphertext range check
+
+  ~DeviceKey() {
+    if (key) {
+      std::lock_guard<std::mutex> lk(Runtime::get().mu);
+      heu_key_destroy(key);
+    }
+  }
+};
+
+namespace {
+
+std::vector<uint32_t> words_of(const BigInt& v, size_t words) {
+  std::vector<uint32_t> w(words + 1);  // one spare word keeps a positive top bit clear
+  v.ToWords(w.data(), words + 1);
+  w.resize(words);
+  return w;
+}
+
+// The reference draws r with key_size/2 bits (encryptor.cc:26-27).
+std::vector<uint32_t> random_r(const DeviceKey& dk, size_t count) {
+  std::vector<uint32_t> r(count * dk.r_words);
+  if (!r.empty() && RAND_bytes(reinterpret_cast<unsigned char*>(r.data()), (int)(r.size() * 4)) != 1)
+    throw_msg("RAND_bytes failed");
+  const unsigned top = (dk.key_bits / 2) % 32;
+  if (top)
+    for (size_t i = 0; i < count; ++i) r[i * dk.r_words + dk.r_words - 1] &= (1u << top) - 1u;
+  return r;
+}
+
+std::shared_ptr<DeviceKey> make_device_key(const BigInt& n, const BigInt& hs, const BigInt* p, const BigInt* q,
+                                           size_t key_bits) {
+  auto dk = std::make_shared<DeviceKey>();
+  dk->key_bits = (uint32_t)key_bits;
+  dk->n_words = (uint32_t)(key_bits / 32);
+  dk->ct_words = 2 * dk->n_words;
+  dk->r_words = (uint32_t)((key_bits / 2 + 31) / 32);
+  auto nw = words_of(n, dk->n_words);
+  auto hw = words_of(hs, dk->ct_words);
+  dk->n2 = words_of(n * n, dk->ct_words);
+  std::vector<uint32_t> pw, qw;
+  if (p && q) {
+    pw = words_of(*p, dk->n_words / 2);
+    qw = words_of(*q, dk->n_words / 2);
+  }
+  Runtime& rt = Runtime::get();
+  {
+    std::lock_guard<std::mutex> lk(rt.mu);
+    ck(heu_key_create(rt.ctx, nw.data(), hw.data(), dk->n_words, p ? pw.data() : nullptr, q ? qw.data() : nullptr,
+                      dk->n_words / 2, dk->key_bits / 2, 0, &dk->key));
+  }
+  dk->has_sk = p && q;
+  return dk;
+}
+
+const DeviceKey& dev(const PublicKey& pk) {
+  if (!pk.dev_) throw_msg("PublicKey is not initialized (call Init())");
+  return *pk.dev_;
+}
+
+// Evaluator VALIDATE (evaluator.cc:20-22): 0 <= c < n^2, packed into rows.
+std::vector<uint32_t> pack(const DeviceKey& dk, ConstSpan<Ciphertext> cts) {
+  const size_t W = dk.ct_words;
+  std::vector<uint32_t> flat(cts.size() * W);
+  for (size_t i = 0; i < cts.size(); ++i) {
+    const auto& c = cts[i]->c_;
+    if (c.size() != W) throw_msg("Evaluator: Invalid ciphertext (wrong limb count)");
+    int cmp = 0;
+    for (size_t j = W; j-- > 0 && cmp == 0;) cmp = c[j] < dk.n2[j] ? -1 : (c[j] > dk.n2[j] ? 1 : 0);
+    if (cmp >= 0) throw_msg("Evaluator: Invalid ciphertext");
+    std::memcpy(flat.data() + i * W, c.data(), W * 4);
+  }
+  return flat;
+}
+
+std::vector<uint32_t> pack(const DeviceKey& dk, Span<Ciphertext> cts) {
+  return pack(dk, ConstSpan<Ciphertext>(cts.data(), cts.size()));
+}
+
+std::vector<Ciphertext> unpack(const DeviceKey& dk, const std::vector<uint32_t>& flat, size_t count) {
+  std::vector<Ciphertext> out(count);
+  const size_t W = dk.ct_words;
+  for (size_t i = 0; i < count; ++i) out[i].c_.assign(flat.begin() + i * W, flat.begin() + (i + 1) * W);
+  return out;
+}
+
+// Plaintexts as int64 when all fit, else two's complement rows of n_words.
+struct PtBatch {
+  bool small = true;
+  std::vector<int64_t> i64;
+  std::vector<uint32_t> rows;
+  uint32_t words = 0;
+};
+
+PtBatch pack_pts(const DeviceKey& dk, ConstSpan<Plaintext> pts, const BigInt* bound, bool negate) {
+  PtBatch b;
+  for (const Plaintext* p : pts) {
+    if (bound && p->CompareAbs(*bound) > 0)
+      throw_msg("plaintext out of range, message=" + p->ToHexString() + ", max (abs)=" + bound->ToHexString());
+    b.small = b.small && p->FitsInt64() && !(negate && p->ToInt64() == std::numeric_limits<int64_t>::min());
+  }
+  if (b.small) {
+    b.i64.reserve(pts.size());
+    for (const Plaintext* p : pts) b.i64.push_back(negate ? -p->ToInt64() : p->ToInt64());
+    return b;
+  }
+  b.words = dk.n_words;
+  b.rows.resize(pts.size() * b.words);
+  for (size_t i = 0; i < pts.size(); ++i) (negate ? -*pts[i] : *pts[i]).ToWords(b.rows.data() + i * b.words, b.words);
+  return b;
+}
+
+template <class T>
+std::vector<const T*> ptrs(const std::vector<T>& v) {
+  std::vector<const T*> out(v.size());
+  for (size_t i = 0; i < v.size(); ++i) out[i] = &v[i];
+  return out;
+}
+
+template <class T>
+ConstSpan<T> cspan(const std::vector<const T*>& v) {
+  return ConstSpan<T>(v.data(), v.size());
+}
+
+void check_same(size_t a, size_t b) {
+  if (a != b) throw_msg("operand spans differ in length: " + std::to_string(a) + " vs " + std::to_string(b));
+}
+
+// ct * (1 + n*p): int64 plaintexts through heu_add_i64; longer ones become
+// (1 + n*p) mod n^2 = Enc(p, r = 0) first, then one ct+ct product.
+std::vector<Ciphertext> add_ct_pt(const DeviceKey& dk, const BigInt& bound, ConstSpan<Ciphertext> a,
+                                  ConstSpan<Plaintext> b, bool negate_b) {
+  check_same(a.size(), b.size());
+  const size_t cnt = a.size();
+  if (cnt == 0) return {};
+  auto ca = pack(dk, a);
+  PtBatch pb = pack_pts(dk, b, &bound, negate_b);
+  std::vector<uint32_t> res(ca.size());
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
+  if (pb.small) {
+    size_t hP = call.in(pb.i64.data(), cnt * 8);
+    call.upload();
+    ck(heu_add_i64(dk.key, call.ptr<uint32_t>(hA), call.ptr<int64_t>(hP), cnt, call.ptr<uint32_t>(hO)));
+  } else {
+    std::vector<uint32_t> rz(cnt, 0u);  // r = 0: h_s^0 = 1
+    size_t hP = call.in(pb.rows.data(), pb.rows.size() * 4), hR = call.in(rz.data(), cnt * 4);
+    size_t hG = call.out(res.size() * 4);
+    call.upload();
+    ck(heu_encrypt(dk.key, call.ptr<uint32_t>(hP), pb.rows.data(), pb.words, call.ptr<uint32_t>(hR), 1, cnt,
+                   call.ptr<uint32_t>(hG)));
+    ck(heu_add(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hG), cnt, call.ptr<uint32_t>(hO)));
+  }
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  return unpack(dk, res, cnt);
+}
+
+template <class F>
+std::vector<Ciphertext> ct_unary(const DeviceKey& dk, ConstSpan<Ciphertext> a, F&& op) {
+  const size_t cnt = a.size();
+  if (cnt == 0) return {};
+  auto ca = pack(dk, a);
+  std::vector<uint32_t> res(ca.size());
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
+  call.upload();
+  ck(op(call.ptr<uint32_t>(hA), cnt, call.ptr<uint32_t>(hO)));
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  return unpack(dk, res, cnt);
+}
+
+template <class F>
+std::vector<Ciphertext> ct_binary(const DeviceKey& dk, ConstSpan<Ciphertext> a, ConstSpan<Ciphertext> b, F&& op) {
+  check_same(a.size(), b.size());
+  const size_t cnt = a.size();
+  if (cnt == 0) return {};
+  auto ca = pack(dk, a), cb = pack(dk, b);
+  std::vector<uint32_t> res(ca.size());
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hB = call.in(cb.data(), cb.size() * 4);
+  size_t hO = call.out(res.size() * 4);
+  call.upload();
+  ck(op(call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hB), cnt, call.ptr<uint32_t>(hO)));
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  return unpack(dk, res, cnt);
+}
+
+template <class T>
+void write_back(Span<T> dst, std::vector<T>&& src) {
+  for (size_t i = 0; i < dst.size(); ++i) *dst[i] = std::move(src[i]);
+}
+
+std::string hex_words(const std::vector<uint32_t>& w) { return BigInt::FromWords(w.data(), w.size(), false).ToHexString(); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ keys
+std::string Ciphertext::ToString() const { return hex_words(c_); }
+
+void PublicKey::Init() {
+  if (key_size_ == 0) key_size_ = n_.BitCount();
+  n_square_ = n_ * n_;
+  n_half_ = n_ >> 1;
+  if (!dev_) dev_ = make_device_key(n_, h_s_, nullptr, nullptr, key_size_);
+}
+
+size_t PublicKey::ct_words() const { return dev(*this).ct_words; }
+size_t PublicKey::n_words() const { return dev(*this).n_words; }
+
+std::string PublicKey::ToString() const {
+  return "paillier_hip PK: n=" + n_.ToHexString() + " (" + std::to_string(key_size_) + " bits), h_s=" +
+         h_s_.ToHexString();
+}
+
+void SecretKey::Init(const PublicKey& pk) {
+  if (pk.dev_ && pk.dev_->has_sk) {
+    dev_ = pk.dev_;
+    return;
+  }
+  dev_ = make_device_key(pk.n_, pk.h_s_, &p_, &q_, pk.key_size_);
+}
+
+std::string SecretKey::ToString() const {
+  return "paillier_hip SK: p=" + p_.ToHexString() + ", q=" + q_.ToHexString();
+}
+
+namespace {
+
+void fill_keys(size_t key_size, const uint64_t* seed, SecretKey* sk, PublicKey* pk) {
+  if (key_size % 64 != 0) throw_msg("Key size must be a multiple of 64");
+  const size_t nw = key_size / 32;
+  std::vector<uint32_t> n(nw), hs(2 * nw), p(nw / 2), q(nw / 2);
+  ck(heu_keygen((uint32_t)key_size, seed, n.data(), hs.data(), p.data(), q.data()));
+  pk->n_ = BigInt::FromWords(n.data(), nw, false);
+  pk->h_s_ = BigInt::FromWords(hs.data(), 2 * nw, false);
+  pk->key_size_ = key_size;
+  sk->p_ = BigInt::FromWords(p.data(), nw / 2, false);
+  sk->q_ = BigInt::FromWords(q.data(), nw / 2, false);
+  // key_generator.cc:50-51: lambda = (p-1)(q-1)/2, mu = lambda^-1 mod n
+  BigInt one(1);
+  sk->lambda_ = ((sk->p_ - one) * (sk->q_ - one)) >> 1;
+  bn_check(BN_mod_inverse(sk->mu_.bn(), sk->lambda_.bn(), pk->n_.bn(), tls_ctx()) != nullptr, "mod_inverse");
+  // one device key (constants, CRT halves, fixed-base table) serves both
+  pk->n_square_ = pk->n_ * pk->n_;
+  pk->n_half_ = pk->n_ >> 1;
+  pk->dev_ = make_device_key(pk->n_, pk->h_s_, &sk->p_, &sk->q_, key_size);
+  sk->dev_ = pk->dev_;
+}
+
+}  // namespace
+
+void KeyGenerator::Generate(size_t key_size, SecretKey* sk, PublicKey* pk) { fill_keys(key_size, nullptr, sk, pk); }
+
+void KeyGenerator::GenerateSeeded(size_t key_size, uint64_t seed, SecretKey* sk, PublicKey* pk) {
+  fill_keys(key_size, &seed, sk, pk);
+}
+
+// ------------------------------------------------------------------ Encryptor
+std::vector<Ciphertext> Encryptor::EncryptZero(int64_t size) const {
+  const DeviceKey& dk = dev(pk_);
+  if (size < 0) throw_msg("EncryptZero: negative size");
+  const size_t cnt = (size_t)size;
+  if (cnt == 0) return {};
+  auto r = random_r(dk, cnt);
+  std::vector<uint32_t> res(cnt * dk.ct_words);
+  DeviceCall call;
+  size_t hR = call.in(r.data(), r.size() * 4), hO = call.out(res.size() * 4);
+  call.upload();
+  ck(heu_randomize(dk.key, nullptr, call.ptr<uint32_t>(hR), dk.r_words, cnt, call.ptr<uint32_t>(hO)));
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  return unpack(dk, res, cnt);
+}
+
+std::vector<Ciphertext> Encryptor::EncryptWithR(ConstSpan<Plaintext> pts, const uint32_t* r, size_t r_words) const {
+  const DeviceKey& dk = dev(pk_);
+  const size_t cnt = pts.size();
+  if (cnt == 0) return {};
+  if (r_words == 0 || !r) throw_msg("EncryptWithR: empty randomness");
+  PtBatch pb = pack_pts(dk, pts, &pk_.n_half_, false);
+  std::vector<uint32_t> res(cnt * dk.ct_words);
+  DeviceCall call;
+  size_t hR = call.in(r, cnt * r_words * 4), hO = call.out(res.size() * 4);
+  if (pb.small) {
+    size_t hM = call.in(pb.i64.data(), cnt * 8);
+    call.upload();
+    ck(heu_encrypt_i64(dk.key, call.ptr<int64_t>(hM), call.ptr<uint32_t>(hR), (uint32_t)r_words, cnt,
+                       call.ptr<uint32_t>(hO)));
+  } else {
+    size_t hM = call.in(pb.rows.data(), pb.rows.size() * 4);
+    call.upload();
+    ck(heu_encrypt(dk.key, call.ptr<uint32_t>(hM), pb.rows.data(), pb.words, call.ptr<uint32_t>(hR),
+                   (uint32_t)r_words, cnt, call.ptr<uint32_t>(hO)));
+  }
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  return unpack(dk, res, cnt);
+}
+
+std::vector<Ciphertext> Encryptor::Encrypt(ConstSpan<Plaintext> pts) const {
+  const DeviceKey& dk = dev(pk_);
+  // range check first (encryptor.cc:38-40), so no randomness is drawn for a bad batch
+  for (const Plaintext* p : pts)
+    if (p->CompareAbs(pk_.n_half_) > 0)
+      throw_msg("message number out of range, message=" + p->ToHexString() + ", max (abs)=" +
+                pk_.n_half_.ToHexString());
+  auto r = random_r(dk, pts.size());
+  return EncryptWithR(pts, r.data(), dk.r_words);
+}
+
+// encryptor.cc:51-66: the audit string records p, rn = h_s^r and c (hex; the
+// engine's values are Z-space, the reference prints its Montgomery form).
+std::pair<std::vector<Ciphertext>, std::vector<std::string>> Encryptor::EncryptWithAudit(
+    ConstSpan<Plaintext> pts) const {
+  const DeviceKey& dk = dev(pk_);
+  const size_t cnt = pts.size();
+  auto r = random_r(dk, cnt);
+  auto cts = EncryptWithR(pts, r.data(), dk.r_words);
+  std::vector<std::string> audit(cnt);
+  if (cnt == 0) return {cts, audit};
+  std::vector<uint32_t> rn(cnt * dk.ct_words);
+  {
+    DeviceCall call;
+    size_t hR = call.in(r.data(), r.size() * 4), hO = call.out(rn.size() * 4);
+    call.upload();
+    ck(heu_randomize(dk.key, nullptr, call.ptr<uint32_t>(hR), dk.r_words, cnt, call.ptr<uint32_t>(hO)));
+    call.download(hO, rn.data(), rn.size() * 4);
+    call.finish();
+  }
+  for (size_t i = 0; i < cnt; ++i) {
+    std::vector<uint32_t> one(rn.begin() + i * dk.ct_words, rn.begin() + (i + 1) * dk.ct_words);
+    audit[i] = "p:" + pts[i]->ToHexString() + ",rn:" + hex_words(one) + ",c:" + cts[i].ToString();
+  }
+  return {cts, audit};
+}
+
+Ciphertext Encryptor::EncryptZero() const { return EncryptZero(1)[0]; }
+
+Ciphertext Encryptor::Encrypt(const Plaintext& m) const {
+  const Plaintext* p = &m;
+  return Encrypt(ConstSpan<Plaintext>(&p, 1))[0];
+}
+
+std::pair<Ciphertext, std::string> Encryptor::EncryptWithAudit(const Plaintext& m) const {
+  const Plaintext* p = &m;
+  auto r = EncryptWithAudit(ConstSpan<Plaintext>(&p, 1));
+  return {r.first[0], r.second[0]};
+}
+
+// ------------------------------------------------------------------ Decryptor
+Decryptor::Decryptor(const PublicKey& pk, const SecretKey& sk) : pk_(pk), sk_(sk) {
+  if (!sk_.dev_ || !sk_.dev_->has_sk) throw_msg("Decryptor: secret key is not initialized (call Init(pk))");
+  if (!(sk_.p_ * sk_.q_ == pk_.n_)) throw_msg("Decryptor: pk and sk are not paired");
+}
+
+void Decryptor::Decrypt(ConstSpan<Ciphertext> in_cts, Span<Plaintext> out_pts) const {
+  check_same(in_cts.size(), out_pts.size());
+  const DeviceKey& dk = *sk_.dev_;
+  const size_t cnt = in_cts.size();
+  if (cnt == 0) return;
+  auto ca = pack(dk, in_cts);
+  const uint32_t ow = dk.n_words;  // |m| <= n/2 always fits n_words in two's complement
+  std::vector<uint32_t> res(cnt * ow);
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
+  call.upload();
+  ck(heu_decrypt(dk.key, call.ptr<uint32_t>(hA), cnt, call.ptr<uint32_t>(hO), ow, nullptr));
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  for (size_t i = 0; i < cnt; ++i) *out_pts[i] = BigInt::FromWords(res.data() + i * ow, ow, true);
+}
+
+std::vector<Plaintext> Decryptor::Decrypt(ConstSpan<Ciphertext> cts) const {
+  std::vector<Plaintext> out(cts.size());
+  std::vector<Plaintext*> op(out.size());
+  for (size_t i = 0; i < out.size(); ++i) op[i] = &out[i];
+  Decrypt(cts, Span<Plaintext>(op.data(), op.size()));
+  return out;
+}
+
+void Decryptor::Decrypt(const Ciphertext& ct, Plaintext* out) const {
+  const Ciphertext* c = &ct;
+  Decrypt(ConstSpan<Ciphertext>(&c, 1), Span<Plaintext>(&out, 1));
+}
+
+Plaintext Decryptor::Decrypt(const Ciphertext& ct) const {
+  Plaintext p;
+  Decrypt(ct, &p);
+  return p;
+}
+
+// ------------------------------------------------------------------ Evaluator
+void Evaluator::Randomize(Span<Ciphertext> ct) const {
+  const DeviceKey& dk = dev(pk_);
+  const size_t cnt = ct.size();
+  if (cnt == 0) return;
+  auto ca = pack(dk, ct);
+  auto r = random_r(dk, cnt);
+  std::vector<uint32_t> res(ca.size());
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hR = call.in(r.data(), r.size() * 4);
+  size_t hO = call.out(res.size() * 4);
+  call.upload();
+  ck(heu_randomize(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hR), dk.r_words, cnt, call.ptr<uint32_t>(hO)));
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  write_back(ct, unpack(dk, res, cnt));
+}
+
+std::vector<Ciphertext> Evaluator::Add(ConstSpan<Ciphertext> a, ConstSpan<Ciphertext> b) const {
+  const DeviceKey& dk = dev(pk_);
+  return ct_binary(dk, a, b, [&](const uint32_t* x, const uint32_t* y, size_t n, uint32_t* o) {
+    return heu_add(dk.key, x, y, n, o);
+  });
+}
+
+std::vector<Ciphertext> Evaluator::Add(ConstSpan<Ciphertext> a, ConstSpan<Plaintext> b) const {
+  return add_ct_pt(dev(pk_), pk_.n_half_, a, b, false);
+}
+
+std::vector<Ciphertext> Evaluator::Add(ConstSpan<Plaintext> a, ConstSpan<Ciphertext> b) const { return Add(b, a); }
+
+std::vector<Plaintext> Evaluator::Add(ConstSpan<Plaintext> a, ConstSpan<Plaintext> b) const {
+  check_same(a.size(), b.size());
+  std::vector<Plaintext> out;
+  out.reserve(a.size());
+  for (size_t i = 0; i < a.size(); ++i) out.push_back(*a[i] + *b[i]);
+  return out;
+}
+
+void Evaluator::AddInplace(Span<Ciphertext> a, ConstSpan<Ciphertext> b) const {
+  write_back(a, Add(ConstSpan<Ciphertext>(a.data(), a.size()), b));
+}
+
+void Evaluator::AddInplace(Span<Ciphertext> a, ConstSpan<Plaintext> b) const {
+  write_back(a, Add(ConstSpan<Ciphertext>(a.data(), a.size()), b));
+}
+
+void Evaluator::AddInplace(Span<Plaintext> a, ConstSpan<Plaintext> b) const {
+  write_back(a, Add(ConstSpan<Plaintext>(a.data(), a.size()), b));
+}
+
+// evaluator.cc:66-80: Sub(a, b) = Add(a, Negate(b)), Sub(ct, p) = Add(ct, -p),
+// Sub(p, ct) = Add(Negate(ct), p).
+std::vector<Ciphertext> Evaluator::Sub(ConstSpan<Ciphertext> a, ConstSpan<Ciphertext> b) const {
+  const DeviceKey& dk = dev(pk_);
+  return ct_binary(dk, a, b, [&](const uint32_t* x, const uint32_t* y, size_t n, uint32_t* o) {
+    return heu_sub(dk.key, x, y, n, o);
+  });
+}
+
+std::vector<Ciphertext> Evaluator::Sub(ConstSpan<Ciphertext> a, ConstSpan<Plaintext> b) const {
+  return add_ct_pt(dev(pk_), pk_.n_half_, a, b, true);
+}
+
+std::vector<Ciphertext> Evaluator::Sub(ConstSpan<Plaintext> a, ConstSpan<Ciphertext> b) const {
+  auto nb = Negate(b);
+  return Add(cspan(ptrs(nb)), a);
+}
+
+std::vector<Plaintext> Evaluator::Sub(ConstSpan<Plaintext> a, ConstSpan<Plaintext> b) const {
+  check_same(a.size(), b.size());
+  std::vector<Plaintext> out;
+  out.reserve(a.size());
+  for (size_t i = 0; i < a.size(); ++i) out.push_back(*a[i] - *b[i]);
+  return out;
+}
+
+void Evaluator::SubInplace(Span<Ciphertext> a, ConstSpan<Ciphertext> b) const {
+  write_back(a, Sub(ConstSpan<Ciphertext>(a.data(), a.size()), b));
+}
+
+void Evaluator::SubInplace(Span<Ciphertext> a, ConstSpan<Plaintext> p) const {
+  write_back(a, Sub(ConstSpan<Ciphertext>(a.data(), a.size()), p));
+}
+
+void Evaluator::SubInplace(Span<Plaintext> a, ConstSpan<Plaintext> b) const {
+  write_back(a, Sub(ConstSpan<Plaintext>(a.data(), a.size()), b));
+}
+
+// evaluator.cc:99-123 (no range check on the scalar: ciphertext overflow is allowed)
+std::vector<Ciphertext> Evaluator::Mul(ConstSpan<Ciphertext> a, ConstSpan<Plaintext> b) const {
+  const DeviceKey& dk = dev(pk_);
+  check_same(a.size(), b.size());
+  const size_t cnt = a.size();
+  if (cnt == 0) return {};
+  bool small = true;
+  size_t bits = 1;
+  for (const Plaintext* p : b) {
+    small = small && p->FitsInt64();
+    bits = std::max(bits, p->BitCount());
+  }
+  auto ca = pack(dk, a);
+  std::vector<uint32_t> res(ca.size());
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
+  std::vector<int64_t> k64;
+  std::vector<uint32_t> mag;
+  std::vector<uint8_t> neg;
+  if (small) {
+    k64.reserve(cnt);
+    for (const Plaintext* p : b) k64.push_back(p->ToInt64());
+    size_t hK = call.in(k64.data(), cnt * 8);
+    call.upload();
+    ck(heu_mul_i64(dk.key, call.ptr<uint32_t>(hA), call.ptr<int64_t>(hK), cnt, call.ptr<uint32_t>(hO)));
+  } else {
+    const uint32_t kw = (uint32_t)((bits + 31) / 32);
+    mag.assign(cnt * kw, 0u);
+    neg.resize(cnt);
+    for (size_t i = 0; i < cnt; ++i) {
+      bn_check(BN_bn2lebinpad(b[i]->bn(), reinterpret_cast<unsigned char*>(mag.data() + i * kw), (int)kw * 4) >= 0,
+               "bn2lebinpad");
+      neg[i] = b[i]->IsNegative() ? 1 : 0;
+    }
+    size_t hK = call.in(mag.data(), mag.size() * 4), hS = call.in(neg.data(), cnt);
+    call.upload();
+    ck(heu_mul(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hK), call.ptr<uint8_t>(hS), kw, cnt,
+               call.ptr<uint32_t>(hO)));
+  }
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  return unpack(dk, res, cnt);
+}
+
+std::vector<Ciphertext> Evaluator::Mul(ConstSpan<Plaintext> a, ConstSpan<Ciphertext> b) const { return Mul(b, a); }
+
+std::vector<Plaintext> Evaluator::Mul(ConstSpan<Plaintext> a, ConstSpan<Plaintext> b) const {
+  check_same(a.size(), b.size());
+  std::vector<Plaintext> out;
+  out.reserve(a.size());
+  for (size_t i = 0; i < a.size(); ++i) out.push_back(*a[i] * *b[i]);
+  return out;
+}
+
+void Evaluator::MulInplace(Span<Ciphertext> a, ConstSpan<Plaintext> b) const {
+  write_back(a, Mul(ConstSpan<Ciphertext>(a.data(), a.size()), b));
+}
+
+void Evaluator::MulInplace(Span<Plaintext> a, ConstSpan<Plaintext> b) const {
+  write_back(a, Mul(ConstSpan<Plaintext>(a.data(), a.size()), b));
+}
+
+std::vector<Ciphertext> Evaluator::Negate(ConstSpan<Ciphertext> a) const {
+  const DeviceKey& dk = dev(pk_);
+  return ct_unary(dk, a, [&](const uint32_t* x, size_t n, uint32_t* o) { return heu_negate(dk.key, x, n, o); });
+}
+
+void Evaluator::NegateInplace(Span<Ciphertext> a) const { write_back(a, Negate(ConstSpan<Ciphertext>(a.data(), a.size()))); }
+
+Ciphertext Evaluator::ReduceSum(ConstSpan<Ciphertext> a) const {
+  const DeviceKey& dk = dev(pk_);
+  auto ca = pack(dk, a);
+  std::vector<uint32_t> res(dk.ct_words);
+  const int64_t offs[2] = {0, (int64_t)a.size()};
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
+  call.upload();
+  ck(heu_sum(dk.key, call.ptr<uint32_t>(hA), offs, 1, call.ptr<uint32_t>(hO)));
+  call.download(hO, res.data(), res.size() * 4);
+  call.finish();
+  return Ciphertext(std::move(res));
+}
+
+// single-element forms
+namespace {
+template <class T>
+ConstSpan<T> one(const T& v, const T*& slot) {
+  slot = &v;
+  return ConstSpan<T>(&slot, 1);
+}
+}  // namespace
+
+void Evaluator::Randomize(Ciphertext* ct) const { Randomize(Span<Ciphertext>(&ct, 1)); }
+
+Ciphertext Evaluator::Add(const Ciphertext& a, const Ciphertext& b) const {
+  const Ciphertext *x, *y;
+  return Add(one(a, x), one(b, y))[0];
+}
+
+Ciphertext Evaluator::Add(const Ciphertext& a, const Plaintext& b) const {
+  const Ciphertext* x;
+  const Plaintext* y;
+  return Add(one(a, x), one(b, y))[0];
+}
+
+void Evaluator::AddInplace(Ciphertext* a, const Ciphertext& b) const { *a = Add(*a, b); }
+void Evaluator::AddInplace(Ciphertext* a, const Plaintext& b) const { *a = Add(*a, b); }
+
+Ciphertext Evaluator::Sub(const Ciphertext& a, const Ciphertext& b) const {
+  const Ciphertext *x, *y;
+  return Sub(one(a, x), one(b, y))[0];
+}
+
+Ciphertext Evaluator::Sub(const Ciphertext& a, const Plaintext& b) const {
+  const Ciphertext* x;
+  const Plaintext* y;
+  return Sub(one(a, x), one(b, y))[0];
+}
+
+Ciphertext Evaluator::Sub(const Plaintext& a, const Ciphertext& b) const {
+  const Plaintext* x;
+  const Ciphertext* y;
+  return Sub(one(a, x), one(b, y))[0];
+}
+
+void Evaluator::SubInplace(Ciphertext* a, const Ciphertext& b) const { *a = Sub(*a, b); }
+void Evaluator::SubInplace(Ciphertext* a, const Plaintext& b) const { *a = Sub(*a, b); }
+
+Ciphertext Evaluator::Mul(const Ciphertext& a, const Plaintext& b) const {
+  const Ciphertext* x;
+  const Plaintext* y;
+  return Mul(one(a, x), one(b, y))[0];
+}
+
+void Evaluator::MulInplace(Ciphertext* a, const Plaintext& b) const { *a = Mul(*a, b); }
+
+Ciphertext Evaluator::Negate(const Ciphertext& a) const {
+  const Ciphertext* x;
+  return Negate(one(a, x))[0];
+}
+
+void Evaluator::NegateInplace(Ciphertext* a) const { *a = Negate(*a); }
+
+}  // namespace paillier_hip
+}  // namespace heu_amd

@@ -116,6 +116,10 @@ int heu_negate(heu_key *key, const uint32_t *a, size_t count, uint32_t *out);
 /* Evaluator::Mul(ct, p) (evaluator.cc:99-123): out = ct^k mod n^2, k = 0 ->
  * 1, k = 1 -> ct, k < 0 -> inverse power. */
 int heu_mul_i64(heu_key *key, const uint32_t *ct, const int64_t *k, size_t count, uint32_t *out);
+/* Same for scalars of any length: |k| as k_words little-endian limbs per
+ * element (k_mag) and its sign (k_neg[i] != 0 for k < 0; k_neg may be NULL). */
+int heu_mul(heu_key *key, const uint32_t *ct, const uint32_t *k_mag, const uint8_t *k_neg, uint32_t k_words,
+            size_t count, uint32_t *out);
 /* numpy Evaluator::Sum per segment (numpy/evaluator.cc:355-373): out[s] =
  * prod ct[offsets[s] .. offsets[s+1]) mod n^2 (1 for an empty segment).
  * offsets: HOST array of segments+1 entries. */

@@ -62,6 +62,7 @@ def lib():
         L.orc_sub.argtypes = [P, P, P, S, P, I]
         L.orc_negate.argtypes = [P, P, S, P, I]
         L.orc_mul_i64.argtypes = [P, P, P, S, P, I]
+        L.orc_mul_twos.argtypes = [P, P, P, I, S, P, I]
         L.orc_randomize.argtypes = [P, P, P, I, S, P, I, I]
         L.orc_dot_i64.argtypes = [P, P, P, P, S, P]
         L.orc_to_mont.argtypes = [P, P, S, I, I, P]
@@ -220,6 +221,15 @@ class Key:
         a, s = np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(s, np.int64)
         out = self._ct(len(a))
         _check(lib().orc_mul_i64(self._h, _ptr(a), _ptr(s), len(a), _ptr(out), threads))
+        return out
+
+    def mul(self, a, scalars, threads=1):
+        """Mul by Python ints of any size (two's complement rows)."""
+        a = np.ascontiguousarray(a, np.uint32)
+        words = max(2, max((abs(int(v)).bit_length() + 32) // 32 for v in scalars))
+        s = np.stack([int_to_twos(int(v), words) for v in scalars]) if len(scalars) else np.zeros((0, words), np.uint32)
+        out = self._ct(len(a))
+        _check(lib().orc_mul_twos(self._h, _ptr(a), _ptr(s), words, len(a), _ptr(out), threads))
         return out
 
     def randomize(self, a, r, use_table=False, threads=1):

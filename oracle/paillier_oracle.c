@@ -691,6 +691,20 @@ static int el_mul_i64(job *j, size_t i, BN_CTX *ctx) {
   return rc;
 }
 
+static int el_mul_twos(job *j, size_t i, BN_CTX *ctx) {
+  const orc_key *k = j->k;
+  size_t w = (size_t)k->ct_words;
+  BIGNUM *a = bn_from_limbs((const uint32_t *)j->a + i * w, w);
+  BIGNUM *p = bn_from_twos((const uint32_t *)j->b + i * (size_t)j->bw, (size_t)j->bw);
+  BIGNUM *o = BN_new();
+  int rc = mul_one(o, k, a, p, ctx);
+  if (!rc) rc = bn_to_limbs(o, (uint32_t *)j->out + i * w, w);
+  BN_free(a);
+  BN_free(p);
+  BN_free(o);
+  return rc;
+}
+
 static int el_randomize(job *j, size_t i, BN_CTX *ctx) {
   /* evaluator.cc:25-28 Randomize: c * h_s^r */
   const orc_key *k = j->k;
@@ -748,6 +762,13 @@ int orc_negate(const orc_key *k, const uint32_t *a, size_t count, uint32_t *out,
 
 int orc_mul_i64(const orc_key *k, const uint32_t *a, const int64_t *s, size_t count, uint32_t *out, int nthreads) {
   job p = {k, el_mul_i64, 0, 0, 0, a, s, NULL, out, 0, 0, 0, 0};
+  return run_batch(p, count, nthreads);
+}
+
+/* Mul with scalars of any length, two's complement over s_words limbs. */
+int orc_mul_twos(const orc_key *k, const uint32_t *a, const uint32_t *s, int s_words, size_t count, uint32_t *out,
+                 int nthreads) {
+  job p = {k, el_mul_twos, 0, 0, 0, a, s, NULL, out, 0, s_words, 0, 0};
   return run_batch(p, count, nthreads);
 }
 

@@ -1,0 +1,411 @@
+// C++ tests of the vectorized-SPI classes (heu_amd/spi/paillier_hip.h) on a
+// HIP device.  The first cases restate the reference's own unit tests,
+// heu/library/algorithms/paillier_zahlen/paillier_test.cc:26-233
+// (CiphertextEvaluate, MinMaxDecrypt, PlaintextEvaluate1/2, BigNumberTest
+// EncDec/SubTest), through the single-element forms; the rest exercise the
+// span (vectorized) forms against closed forms computed with OpenSSL on the
+// host.  Prints one line per case and "ALL PASSED" at the end; exit status 0
+// iff every check held.
+#include <openssl/bn.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <functional>
+#include <limits>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "paillier_hip.h"
+
+using heu_amd::BigInt;
+using heu_amd::ConstSpan;
+using heu_amd::Span;
+using namespace heu_amd::paillier_hip;
+
+namespace {
+
+int g_failures = 0;
+
+#define EXPECT_TRUE(c)                                                          \
+  do {                                                                          \
+    if (!(c)) {                                                                 \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #c);              \
+      ++g_failures;                                                             \
+    }                                                                           \
+  } while (0)
+#define EXPECT_FALSE(c) EXPECT_TRUE(!(c))
+#define EXPECT_EQ(a, b) EXPECT_TRUE((a) == (b))
+#define EXPECT_THROW(stmt)                                                      \
+  do {                                                                          \
+    bool thrown_ = false;                                                       \
+    try {                                                                       \
+      stmt;                                                                     \
+    } catch (const std::exception&) {                                           \
+      thrown_ = true;                                                           \
+    }                                                                           \
+    if (!thrown_) {                                                             \
+      std::printf("  FAILED %s:%d: no exception from %s\n", __FILE__, __LINE__, #stmt); \
+      ++g_failures;                                                             \
+    }                                                                           \
+  } while (0)
+
+struct Keys {
+  SecretKey sk;
+  PublicKey pk;
+};
+
+void run(const char* name, const std::function<void()>& body) {
+  std::printf("[ RUN  ] %s\n", name);
+  std::fflush(stdout);
+  int before = g_failures;
+  try {
+    body();
+  } catch (const std::exception& e) {
+    std::printf("  FAILED: exception %s\n", e.what());
+    ++g_failures;
+  }
+  std::printf("[ %s ] %s\n", g_failures == before ? " OK " : "FAIL", name);
+  std::fflush(stdout);
+}
+
+template <class T>
+std::vector<const T*> cptrs(const std::vector<T>& v) {
+  std::vector<const T*> o;
+  for (auto& x : v) o.push_back(&x);
+  return o;
+}
+
+template <class T>
+std::vector<T*> mptrs(std::vector<T>& v) {
+  std::vector<T*> o;
+  for (auto& x : v) o.push_back(&x);
+  return o;
+}
+
+template <class T>
+ConstSpan<T> cs(const std::vector<const T*>& v) {
+  return ConstSpan<T>(v.data(), v.size());
+}
+
+template <class T>
+Span<T> mspan(const std::vector<T*>& v) {
+  return Span<T>(v.data(), v.size());
+}
+
+BigInt ct_value(const Ciphertext& c) { return BigInt::FromWords(c.c_.data(), c.c_.size(), false); }
+
+BigInt mod_exp(const BigInt& b, const BigInt& e, const BigInt& m) {
+  BigInt r;
+  BN_CTX* ctx = BN_CTX_new();
+  BN_mod_exp(r.bn(), b.bn(), e.bn(), m.bn(), ctx);
+  BN_CTX_free(ctx);
+  return r;
+}
+
+BigInt mod_mul(const BigInt& a, const BigInt& b, const BigInt& m) {
+  BigInt r;
+  BN_CTX* ctx = BN_CTX_new();
+  BN_mod_mul(r.bn(), a.bn(), b.bn(), m.bn(), ctx);
+  BN_CTX_free(ctx);
+  return r;
+}
+
+BigInt nnmod(const BigInt& a, const BigInt& m) {
+  BigInt r;
+  BN_CTX* ctx = BN_CTX_new();
+  BN_nnmod(r.bn(), a.bn(), m.bn(), ctx);
+  BN_CTX_free(ctx);
+  return r;
+}
+
+BigInt random_bits(std::mt19937_64& g, int bits, bool allow_negative) {
+  std::vector<uint32_t> w((bits + 31) / 32);
+  for (auto& x : w) x = (uint32_t)g();
+  if (bits % 32) w.back() &= (1u << (bits % 32)) - 1u;
+  BigInt v = BigInt::FromWords(w.data(), w.size(), false);
+  return allow_negative && (g() & 1) ? -v : v;
+}
+
+// paillier_test.cc:26-96
+void CiphertextEvaluate(const Keys& k) {
+  Encryptor encryptor(k.pk);
+  Evaluator evaluator(k.pk);
+  Decryptor decryptor(k.pk, k.sk);
+  EXPECT_TRUE(ct_value(encryptor.EncryptZero()) > BigInt(1));
+
+  BigInt m0(-12345);
+  Ciphertext ct0 = encryptor.Encrypt(m0);
+  BigInt plain;
+  Ciphertext res = evaluator.Add(ct0, ct0);
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(-12345 * 2));
+  res = evaluator.Mul(ct0, BigInt(2));
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(-12345 * 2));
+  evaluator.Randomize(&res);
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(-12345 * 2));
+
+  BigInt m1(123);
+  Ciphertext ct1 = encryptor.Encrypt(m1);
+  res = evaluator.Add(ct1, ct1);
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(123 * 2));
+  res = evaluator.Mul(ct1, BigInt(2));
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(123 * 2));
+  res = evaluator.Add(ct0, ct1);
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(-12345 + 123));
+
+  ct1 = encryptor.Encrypt(m1);
+  res = evaluator.Mul(ct1, BigInt(1));
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(123));
+
+  ct1 = encryptor.Encrypt(m1);
+  res = evaluator.Mul(ct1, BigInt(0));
+  decryptor.Decrypt(res, &plain);
+  EXPECT_TRUE(plain.IsZero());
+  evaluator.Randomize(&res);
+  EXPECT_FALSE(ct_value(res).IsZero());
+  decryptor.Decrypt(res, &plain);
+  EXPECT_TRUE(plain.IsZero());
+
+  ct1 = encryptor.Encrypt(m1);
+  res = evaluator.Mul(ct1, BigInt(-1));
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(-123));
+
+  ct1 = encryptor.Encrypt(m1);
+  res = evaluator.Mul(ct1, BigInt(-2));
+  decryptor.Decrypt(res, &plain);
+  EXPECT_EQ(plain, BigInt(-123 * 2));
+}
+
+// paillier_test.cc:98-120
+void MinMaxDecrypt(const Keys& k) {
+  Encryptor encryptor(k.pk);
+  Decryptor decryptor(k.pk, k.sk);
+  BigInt plain = k.pk.n_;
+  EXPECT_THROW(encryptor.Encrypt(plain));
+  plain = k.pk.PlaintextBound() + BigInt(1);
+  EXPECT_THROW(encryptor.Encrypt(plain));
+
+  BigInt plain2;
+  plain = plain - BigInt(1);  // max
+  Ciphertext ct0 = encryptor.Encrypt(plain);
+  decryptor.Decrypt(ct0, &plain2);
+  EXPECT_EQ(plain, plain2);
+
+  plain = -plain;  // -max
+  ct0 = encryptor.Encrypt(plain);
+  decryptor.Decrypt(ct0, &plain2);
+  EXPECT_EQ(plain, plain2);
+
+  plain = plain - BigInt(1);
+  EXPECT_THROW(encryptor.Encrypt(plain));
+}
+
+// paillier_test.cc:122-186
+void PlaintextEvaluate(const Keys& k) {
+  Encryptor encryptor(k.pk);
+  Evaluator evaluator(k.pk);
+  Decryptor decryptor(k.pk, k.sk);
+  const int64_t kMax = std::numeric_limits<int64_t>::max(), kMin = std::numeric_limits<int64_t>::lowest();
+
+  Ciphertext ct0 = encryptor.Encrypt(BigInt(123));
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(23))), BigInt(123 + 23));
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(6543212))), BigInt(123 + 6543212));
+  EXPECT_TRUE(decryptor.Decrypt(evaluator.Add(ct0, BigInt(-123))).IsZero());
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(-456))), BigInt(123 - 456));
+
+  ct0 = encryptor.Encrypt(BigInt(-123));
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(23))), BigInt(-123 + 23));
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(kMax))), BigInt(-123 + kMax));
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(-123))), BigInt(-123 * 2));
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(-456))), BigInt(-123 - 456));
+
+  ct0 = encryptor.Encrypt(BigInt(kMin));
+  EXPECT_EQ(decryptor.Decrypt(evaluator.Add(ct0, BigInt(kMax))), BigInt(-1));
+  // kMin - 1 no longer fits int64 but is an ordinary plaintext
+  BigInt below = decryptor.Decrypt(evaluator.Add(ct0, BigInt(-1)));
+  EXPECT_EQ(below, BigInt(kMin) - BigInt(1));
+  EXPECT_FALSE(below.FitsInt64());
+}
+
+// paillier_test.cc:188-233 (BigNumberTest EncDec / SubTest over the listed values)
+void BigNumberEncDecSub(const Keys& k) {
+  Encryptor encryptor(k.pk);
+  Evaluator evaluator(k.pk);
+  Decryptor decryptor(k.pk, k.sk);
+  const int64_t vals[] = {std::numeric_limits<int64_t>::lowest(), -1234, -2, -1, 0, 1, 2, 1234,
+                          std::numeric_limits<int64_t>::max()};
+  for (int64_t x : vals) {
+    BigInt pt1(x);
+    EXPECT_EQ(decryptor.Decrypt(encryptor.Encrypt(pt1)), pt1);
+
+    const int64_t share_a = std::numeric_limits<int64_t>::max();
+    Ciphertext xe = encryptor.Encrypt(BigInt(x));
+    auto xa = encryptor.EncryptWithAudit(BigInt(x));
+    EXPECT_EQ(decryptor.Decrypt(xa.first), BigInt(x));
+    EXPECT_TRUE(xa.second.rfind("p:", 0) == 0 && xa.second.find(",rn:") != std::string::npos &&
+                xa.second.find(",c:" + xa.first.ToString()) != std::string::npos);
+    evaluator.SubInplace(&xe, BigInt(share_a));
+    BigInt raw = decryptor.Decrypt(xe);
+    // share_a + share_b == x in int64 arithmetic (wrapping, as the reference's Get<int64_t>)
+    uint64_t share_b = raw.FitsInt64() ? (uint64_t)raw.ToInt64() : 0;
+    if (!raw.FitsInt64()) {
+      uint32_t w[4];
+      raw.ToWords(w, 4);
+      share_b = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+    }
+    EXPECT_EQ((int64_t)((uint64_t)share_a + share_b), x);
+  }
+}
+
+// Span forms: EncryptWithR against (1 + n m) h_s^r mod n^2 computed on the
+// host, then every evaluator op against its closed form / decryption.
+void VectorizedClosedForms(const Keys& k, size_t count) {
+  Encryptor encryptor(k.pk);
+  Evaluator evaluator(k.pk);
+  Decryptor decryptor(k.pk, k.sk);
+  const BigInt& n = k.pk.n_;
+  const BigInt& n2 = k.pk.n_square_;
+  std::mt19937_64 g(42 + count);
+  const int nbits = (int)n.BitCount();
+
+  std::vector<BigInt> ms, ms2, ks;
+  for (size_t i = 0; i < count; ++i) {
+    int bits = (int)(g() % (uint64_t)(nbits - 2)) + 1;
+    ms.push_back(i % 3 == 0 ? BigInt((int64_t)g()) : random_bits(g, bits, true));
+    ms2.push_back(BigInt((int64_t)(g() >> 1)) - BigInt((int64_t)(g() >> 1)));
+    ks.push_back(i % 2 ? BigInt((int64_t)(int32_t)g()) : random_bits(g, (int)(g() % 300) + 1, true));
+  }
+  ms[0] = k.pk.n_half_;
+  ms[1] = -k.pk.n_half_;
+  ms[2] = BigInt(0);
+  ks[0] = BigInt(0);
+  ks[1] = BigInt(1);
+  ks[2] = BigInt(-1);
+
+  const size_t rw = k.pk.key_size_ / 64;
+  std::vector<uint32_t> r(count * rw);
+  for (auto& x : r) x = (uint32_t)g();
+  auto pm = cptrs(ms), pm2 = cptrs(ms2), pk = cptrs(ks);
+  auto ct = encryptor.EncryptWithR(cs(pm), r.data(), rw);
+  bool enc_ok = true;
+  for (size_t i = 0; i < count && enc_ok; ++i) {
+    BigInt ri = BigInt::FromWords(r.data() + i * rw, rw, false);
+    BigInt gm = nnmod(n * ms[i] + BigInt(1), n2);
+    enc_ok = ct_value(ct[i]) == mod_mul(gm, mod_exp(k.pk.h_s_, ri, n2), n2);
+  }
+  EXPECT_TRUE(enc_ok);
+
+  auto pc = cptrs(ct);
+  auto dec = decryptor.Decrypt(cs(pc));
+  bool dec_ok = true;
+  for (size_t i = 0; i < count; ++i) dec_ok = dec_ok && dec[i] == ms[i];
+  EXPECT_TRUE(dec_ok);
+
+  // ct x big scalar: c^k mod n^2 (inverse power for k < 0)
+  auto prod = evaluator.Mul(cs(pc), cs(pk));
+  bool mul_ok = true;
+  for (size_t i = 0; i < count && mul_ok; ++i) {
+    BigInt e = ks[i].IsNegative() ? -ks[i] : ks[i];
+    BigInt want = mod_exp(ct_value(ct[i]), e, n2);
+    if (ks[i].IsNegative()) BN_mod_inverse(want.bn(), want.bn(), n2.bn(), nullptr);
+    mul_ok = ct_value(prod[i]) == want;
+  }
+  EXPECT_TRUE(mul_ok);
+
+  // ct + int64 pt, ct - pt, pt - ct, ct + ct, negate: decrypt to the plaintext arithmetic
+  auto ct2 = encryptor.Encrypt(cs(pm2));
+  auto pc2 = cptrs(ct2);
+  auto dsum = decryptor.Decrypt(cs(cptrs(evaluator.Add(cs(pc), cs(pc2)))));
+  auto dadd = decryptor.Decrypt(cs(cptrs(evaluator.Add(cs(pc), cs(pm2)))));
+  auto dsub = decryptor.Decrypt(cs(cptrs(evaluator.Sub(cs(pc), cs(pc2)))));
+  auto dsubp = decryptor.Decrypt(cs(cptrs(evaluator.Sub(cs(pm2), cs(pc)))));
+  auto dneg = decryptor.Decrypt(cs(cptrs(evaluator.Negate(cs(pc)))));
+  auto wrap = [&](const BigInt& v) {  // centered residue mod n
+    BigInt m = nnmod(v, n);
+    return m > k.pk.n_half_ ? m - n : m;
+  };
+  bool arith_ok = true;
+  for (size_t i = 0; i < count; ++i) {
+    arith_ok = arith_ok && dsum[i] == wrap(ms[i] + ms2[i]) && dadd[i] == wrap(ms[i] + ms2[i]) &&
+               dsub[i] == wrap(ms[i] - ms2[i]) && dsubp[i] == wrap(ms2[i] - ms[i]) && dneg[i] == wrap(-ms[i]);
+  }
+  EXPECT_TRUE(arith_ok);
+
+  // ct + big pt goes through Enc(p, r = 0)
+  auto dbig = decryptor.Decrypt(cs(cptrs(evaluator.Add(cs(pc), cs(pm)))));
+  bool big_ok = true;
+  for (size_t i = 0; i < count; ++i) big_ok = big_ok && dbig[i] == wrap(ms[i] + ms[i]);
+  EXPECT_TRUE(big_ok);
+
+  // in-place forms and ReduceSum
+  auto acc = ct2;
+  auto macc = mptrs(acc);
+  evaluator.AddInplace(mspan(macc), cs(pc));
+  evaluator.MulInplace(mspan(macc), cs(pk));
+  evaluator.Randomize(mspan(macc));
+  auto dacc = decryptor.Decrypt(cs(cptrs(acc)));
+  bool inplace_ok = true;
+  for (size_t i = 0; i < count; ++i) inplace_ok = inplace_ok && dacc[i] == wrap((ms[i] + ms2[i]) * ks[i]);
+  EXPECT_TRUE(inplace_ok);
+  BigInt total(0);
+  for (auto& v : ms2) total = total + v;
+  EXPECT_EQ(decryptor.Decrypt(evaluator.ReduceSum(cs(pc2))), wrap(total));
+
+  // EncryptZero batch decrypts to zeros
+  auto z = encryptor.EncryptZero((int64_t)count);
+  bool zero_ok = true;
+  for (auto& p : decryptor.Decrypt(cs(cptrs(z)))) zero_ok = zero_ok && p.IsZero();
+  EXPECT_TRUE(zero_ok);
+
+  // an invalid ciphertext (>= n^2) is rejected like evaluator.cc:20-22
+  Ciphertext bad = ct[0];
+  std::vector<uint32_t> n2w(bad.c_.size() + 1);
+  n2.ToWords(n2w.data(), n2w.size());
+  bad.c_.assign(n2w.begin(), n2w.end() - 1);
+  EXPECT_THROW(evaluator.Negate(bad));
+  // span length mismatch
+  std::vector<const Ciphertext*> shorter(pc.begin(), pc.end() - 1);
+  EXPECT_THROW(evaluator.Add(cs(pc), cs(shorter)));
+}
+
+}  // namespace
+
+int main() {
+  Keys k2048, k3072, seeded;
+  run("KeyGenerator.Generate(2048)", [&] {
+    KeyGenerator::Generate(2048, &k2048.sk, &k2048.pk);
+    EXPECT_EQ(k2048.pk.n_.BitCount(), (size_t)2048);
+    EXPECT_TRUE(k2048.sk.p_ * k2048.sk.q_ == k2048.pk.n_);
+  });
+  run("KeyGenerator.GenerateSeeded(2048, 0)", [&] {
+    KeyGenerator::GenerateSeeded(2048, 0, &seeded.sk, &seeded.pk);
+    Keys again;
+    KeyGenerator::GenerateSeeded(2048, 0, &again.sk, &again.pk);
+    EXPECT_TRUE(seeded.pk == again.pk);
+    EXPECT_TRUE(seeded.sk == again.sk);
+    std::printf("  seeded n = %s\n", seeded.pk.n_.ToHexString().c_str());
+  });
+  run("ZPaillierTest.CiphertextEvaluate", [&] { CiphertextEvaluate(k2048); });
+  run("ZPaillierTest.MinMaxDecrypt", [&] { MinMaxDecrypt(k2048); });
+  run("ZPaillierTest.PlaintextEvaluate", [&] { PlaintextEvaluate(k2048); });
+  run("BigNumberTest.EncDec+SubTest", [&] { BigNumberEncDecSub(k2048); });
+  run("Vectorized.ClosedForms(2048, 257)", [&] { VectorizedClosedForms(seeded, 257); });
+  run("KeyGenerator.Generate(3072)", [&] { KeyGenerator::Generate(3072, &k3072.sk, &k3072.pk); });
+  run("ZPaillierTest.MinMaxDecrypt(3072)", [&] { MinMaxDecrypt(k3072); });
+  run("Vectorized.ClosedForms(3072, 65)", [&] { VectorizedClosedForms(k3072, 65); });
+  run("Decryptor rejects an unpaired key", [&] { EXPECT_THROW(Decryptor(k2048.pk, k3072.sk)); });
+  if (g_failures) {
+    std::printf("%d FAILURE(S)\n", g_failures);
+    return 1;
+  }
+  std::printf("ALL PASSED\n");
+  return 0;
+}

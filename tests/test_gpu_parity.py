@@ -112,6 +112,32 @@ def test_mul_scalar(eng, which, request):
 
 
 @pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_mul_big_scalar(eng, which, request):
+    # Mul(ct, pt) with plaintext scalars of any length (evaluator.cc:99-123):
+    # special cases 0 / 1 / -1, int64 edges, and scalars up to n/2.
+    kd, gk, ok = request.getfixturevalue(which)
+    m, r = inputs(eng, 40, 12)
+    a = ok.encrypt_i64(m, r)
+    rng = np.random.default_rng(13)
+    half = kd["n"] >> 1
+    ks = [0, 1, -1, 2, -2, I64_MAX, I64_MIN, 2**64, -(2**64) - 1, half, -half, 2**100 + 3]
+    while len(ks) < 40:
+        bits = int(rng.integers(1, kd["n"].bit_length() - 1))
+        v = int.from_bytes(rng.bytes((bits + 7) // 8), "little") & ((1 << bits) - 1)
+        ks.append(-v if rng.integers(2) else v)
+    print(f"[{which}] mul (big scalars) n=40", flush=True)
+    out = gk.mul(a, ks)
+    gk.ctx.sync()
+    assert np.array_equal(host(out), ok.mul(a, ks))
+    print(f"[{which}] mul (short scalars == mul_i64) n=40", flush=True)
+    small = rng.integers(-(2**40), 2**40, size=40, dtype=np.int64)
+    small[:3] = [0, 1, -1]
+    out2 = gk.mul(a, small.tolist())
+    gk.ctx.sync()
+    assert np.array_equal(host(out2), ok.mul_i64(a, small))
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
 def test_negate_sub(eng, which, request):
     kd, gk, ok = request.getfixturevalue(which)
     m, r = inputs(eng, 40, 6)

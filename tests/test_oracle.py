@@ -140,6 +140,25 @@ def test_mul_special_cases_and_dot(oracle_mod):
     assert key.decrypt(O.ints_to_rows([dot[0]], key.ct_words), out_words=2).view(np.int64)[0, 0] == 15 + 14
 
 
+def test_mul_big_scalars_closed_form(oracle_mod):
+    # Mul by scalars wider than 64 bits: c^k mod n^2, inverse power for k < 0,
+    # and Dec(Mul(Enc(m), k)) = m*k when |m*k| <= n/2.
+    O = oracle_mod
+    kd = O.keygen(512, 9)
+    key = O.Key.from_keygen(kd)
+    n, n2 = kd["n"], kd["n"] ** 2
+    ks = [0, 1, -1, 2**64, -(2**64) - 5, 2**200 + 12345, -(2**230), (n >> 1) - 1]
+    ms = [3, -4, 5, -7, 11, 1, -1, 1]
+    ct = key.encrypt_i64(np.array(ms, np.int64), O.ints_to_rows(list(range(3, 3 + len(ms))), 8))
+    c = O.rows_to_ints(ct)
+    out = key.mul(ct, ks)
+    assert O.rows_to_ints(out) == [pow(ci, k, n2) for ci, k in zip(c, ks)]
+    dec = [O.twos_to_int(r) for r in key.decrypt(out, out_words=key.ct_words // 2)]
+    assert dec == [m * k for m, k in zip(ms, ks)]
+    small = np.array([2, -3, 0, 1, -1, 7, 9, -9], np.int64)
+    assert np.array_equal(key.mul(ct, small.tolist()), key.mul_i64(ct, small))
+
+
 def test_mont_form_export(oracle_mod):
     O = oracle_mod
     kd = O.keygen(512, 8)
