@@ -44,7 +44,22 @@ SIGNATURES = {
     "heu_dot_i64": (I, [P, P, P, P, SZ, P]),
     "heu_to_mont": (I, [P, P, SZ, U32, P]),
     "heu_from_mont": (I, [P, P, SZ, U32, P]),
+    "heu_batch_create": (I, [P, P, SZ, U32, ctypes.POINTER(P)]),
+    "heu_batch_run": (I, [P]),
+    "heu_batch_destroy": (I, [P]),
+    "heu_batch_stats": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(SZ)]),
+    "heu_ctx_set_grid_multiplier": (I, [P, U32, I, U32]),
 }
+
+# heu_batch ops and the heu_task struct (include/heu_paillier.h)
+(HEU_OP_ENCRYPT_I64, HEU_OP_RANDOMIZE, HEU_OP_ADD, HEU_OP_ADD_I64, HEU_OP_MUL_I64, HEU_OP_DECRYPT, HEU_OP_SUM,
+ HEU_OP_DOT_I64, HEU_OP_BARRIER) = range(1, 10)
+HEU_BATCH_GRAPH = 1
+
+
+class HeuTask(ctypes.Structure):
+    _fields_ = [("key", P), ("op", ctypes.c_int32), ("aux", ctypes.c_uint32), ("a", P), ("b", P),
+                ("count", SZ), ("offsets", P), ("segments", SZ), ("out", P)]
 
 _lib = None
 _lock = threading.Lock()

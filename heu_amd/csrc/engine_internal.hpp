@@ -1,0 +1,166 @@
+// engine_internal.hpp -- engine state shared by the C ABI translation units
+// (engine.cpp: contexts, keys and one-op entry points; batch.cpp: multi-task
+// batches and their graphs).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/heu_paillier.h"
+#include "kernels.hpp"
+
+namespace heu_engine {
+
+// Records the message of a failing call (heu_last_error) and returns `code`.
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return ::heu_engine::fail(e_ == hipErrorOutOfMemory ? HEU_ENOMEM : HEU_EDEVICE, "%s: %s", #expr, \
+                                hipGetErrorString(e_));                                            \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) return HEU_OK;
+    HIP_TRY(hipMalloc(&p, n));
+    bytes = n;
+    return HEU_OK;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+constexpr int kInvChunk = 32;
+
+// Element counts per level of a batch inversion (Montgomery's trick over
+// chunks of kInvChunk, up to one value).
+std::vector<long> inv_levels(long count);
+
+// Chunking of segments for a reduction level.
+std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, const std::vector<long>& seg_len, long chunk,
+                                        std::vector<long>* seg_first_chunk, std::vector<long>* seg_nchunks);
+
+}  // namespace heu_engine
+
+using heu_hip::CombArgs;
+using heu_hip::DecArgs;
+using heu_hip::ModArgs;
+using heu_hip::Ops;
+using heu_hip::PubArgs;
+
+struct heu_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  uint32_t* d_err = nullptr;
+  std::mutex mu;
+  // scratch pool: one growable buffer carved per call
+  heu_engine::DevBuf scratch;
+  size_t scratch_used = 0;
+  // small host->device staging for chunk lists etc.
+  heu_engine::DevBuf meta;
+  // optional per-kernel timing: HIP events recorded on the work stream
+  struct ProfRec {
+    const char* name;
+    hipEvent_t start, stop;
+  };
+  // persistent work-list launches: grid multiplier per [key size is 3072][decryption shape]
+  // and the occupancy of each work-list kernel (resident workgroups per CU)
+  uint32_t grid_mult[2][2] = {{1, 1}, {1, 1}};
+  std::map<const heu_hip::WorklistShape*, int> wl_resident;
+  bool prof = false;
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_pool;
+  hipEvent_t prof_event() {
+    if (!prof_pool.empty()) {
+      hipEvent_t e = prof_pool.back();
+      prof_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  void prof_clear() {
+    for (auto& r : prof_recs) {
+      prof_pool.push_back(r.start);
+      prof_pool.push_back(r.stop);
+    }
+    prof_recs.clear();
+  }
+};
+
+// Launches `call` and, when profiling is on, brackets it with events.
+#define LAUNCH(ctx, name, call)                             \
+  do {                                                      \
+    hipEvent_t a_ = nullptr;                                \
+    if ((ctx)->prof) {                                      \
+      a_ = (ctx)->prof_event();                             \
+      (void)hipEventRecord(a_, (ctx)->stream);              \
+    }                                                       \
+    call;                                                   \
+    if ((ctx)->prof) {                                      \
+      hipEvent_t b_ = (ctx)->prof_event();                  \
+      (void)hipEventRecord(b_, (ctx)->stream);              \
+      (ctx)->prof_recs.push_back({(name), a_, b_});         \
+    }                                                       \
+  } while (0)
+
+struct heu_key {
+  heu_ctx* ctx = nullptr;
+  const Ops* ops = nullptr;
+  int key_bits = 0, n_words = 0, ct_words = 0, prime_words = 0, has_sk = 0;
+  int nl_n = 0, nl_p = 0;
+  // public side
+  heu_engine::DevBuf n2_m, n2_one, n2_r2, n29, table, n2_words;
+  uint32_t n2_mprime = 0;
+  int windows = 0, unit = 0;
+  // secret side
+  heu_engine::DevBuf p2_m, p2_one, p2_r2, p2_r3, q2_m, q2_one, q2_r2, q2_r3, sched_p, sched_q;
+  uint32_t p2_mprime = 0, q2_mprime = 0;
+  int nsched_p = 0, nsched_q = 0;
+  heu_engine::DevBuf c_p, c_q, c_pinv, c_qinv, c_hpR, c_hqR, c_pinvqR, c_n, c_nhalf;
+  uint32_t c_pprime = 0, c_qprime = 0;
+  // host copies for argument checks and conversions
+  std::vector<uint32_t> n_host, n2_host, nhalf_host;
+  size_t device_bytes = 0;
+
+  ModArgs mod_n2() const {
+    return ModArgs{n2_m.as<uint32_t>(), n2_one.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_mprime};
+  }
+  PubArgs pub() const { return PubArgs{mod_n2(), n29.as<uint32_t>(), table.as<uint32_t>(), windows, unit}; }
+  DecArgs dec() const {
+    return DecArgs{ModArgs{p2_m.as<uint32_t>(), p2_one.as<uint32_t>(), p2_r2.as<uint32_t>(), p2_r3.as<uint32_t>(),
+                           p2_mprime},
+                   ModArgs{q2_m.as<uint32_t>(), q2_one.as<uint32_t>(), q2_r2.as<uint32_t>(), q2_r3.as<uint32_t>(),
+                           q2_mprime},
+                   sched_p.as<uint32_t>(), sched_q.as<uint32_t>(), nsched_p, nsched_q};
+  }
+  CombArgs comb() const {
+    return CombArgs{c_p.as<uint32_t>(),   c_q.as<uint32_t>(),    c_pinv.as<uint32_t>(), c_qinv.as<uint32_t>(),
+                    c_hpR.as<uint32_t>(), c_hqR.as<uint32_t>(),  c_pinvqR.as<uint32_t>(), c_n.as<uint32_t>(),
+                    c_nhalf.as<uint32_t>(), c_pprime, c_qprime};
+  }
+};

@@ -52,6 +52,35 @@ struct Chunk {
   long start, len;
 };
 
+// Work-list launches: one persistent launch serves many items (vectors of any
+// length, of any key of the launch's shape).  Units are blocks of groups; a
+// workgroup takes the next unit from an atomic counter, finds its item by
+// binary search over unit0 and runs the op's per-element body.
+enum WorkOp : int {
+  kWorkEncrypt = 0,  // a: int64 m, b: r (aux words each), out: ct
+  kWorkRandomize,    // a: ct or null (EncryptZero), b: r (aux words), out: ct
+  kWorkAdd,          // a, b: ct, out: ct
+  kWorkAddPt,        // a: ct, b: int64 pt, out: ct
+  kWorkMul,          // a: ct, b: int64 scalar, out: ct (|k| power), neg: sign flags
+  kWorkDot0,         // a: ct, b: int64 weights, c: chunks, out/out2: pos/neg partials (count = chunks)
+  kWorkSum0,         // a: ct, c: chunks, out: partials (count = chunks)
+  kWorkDecHalf       // a: ct, c: window schedule (aux ops), out: this half's outputs; pub.n2 = p^2 or q^2
+};
+
+struct WorkItem {
+  long unit0;  // first unit of this item in the launch (items sorted by unit0)
+  long count;  // elements, or chunks for the reductions
+  int op, aux;
+  PubArgs pub;  // pub.n2 is the item's modulus
+  const void* a;
+  const void* b;
+  const void* c;
+  void* out;
+  void* out2;
+  u8* neg;
+  uint32_t* err;
+};
+
 typedef void (*TableLevelFn)(hipStream_t, const ModArgs&, const uint32_t*, uint32_t*, int, int, int);
 typedef void (*EncI64Fn)(hipStream_t, const PubArgs&, const int64_t*, const uint32_t*, int, long, uint32_t*,
                          uint32_t*);
@@ -78,6 +107,20 @@ typedef void (*InvSingleFn)(hipStream_t, const ModArgs&, const uint32_t*, uint32
 typedef void (*MulConstFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
 typedef void (*DecHalfFn)(hipStream_t, const DecArgs&, const uint32_t*, long, uint32_t*, uint32_t*);
 typedef void (*CombineFn)(hipStream_t, const CombArgs&, const uint32_t*, long, uint32_t*, int, u8*);
+// items (device), item count, total units, counters (2 words, zero between
+// launches: the kernel's last workgroup clears them), per-slot window tables,
+// grid size (workgroups)
+typedef void (*WorklistFn)(hipStream_t, const WorkItem*, int, uint32_t, uint32_t*, uint32_t*, unsigned);
+// Resident workgroups per CU of a work-list kernel (occupancy query).
+typedef int (*ResidentFn)();
+
+// Geometry of one work-list kernel.
+struct WorklistShape {
+  WorklistFn launch;
+  ResidentFn resident;
+  int groups_per_unit;      // elements per unit (kBlock / L)
+  long slot_table_words;    // window-table words per group slot
+};
 
 // Launchers and shape constants for one key size.
 struct Ops {
@@ -102,6 +145,8 @@ struct Ops {
   MulConstFn mul_const;
   DecHalfFn dec_half;
   CombineFn combine;
+  WorklistShape wl_n;  // n^2 ops: encrypt, randomize, add, add_pt, mul, dot0, sum0
+  WorklistShape wl_p;  // CRT decryption halves mod p^2 / q^2
 };
 
 const Ops* ops_for_key_bits(int key_bits);

@@ -1,4 +1,4 @@
-// paillier_kernels.hip -- gfx950 kernels of the Paillier batch engine.
+// paillier_device.hpp -- gfx950 kernels of the Paillier batch engine.
 //
 // Every kernel works on residues spread over lane groups (mp_group.hpp).  One
 // group handles one element (or one chunk of a reduction); a block holds
@@ -8,6 +8,7 @@
 // the Montgomery working set, which is what lets the wide shapes (K = 27, 36)
 // run without spilling.  Kernels never allocate: scratch buffers come from the
 // host engine.
+#pragma once
 #include "kernels.hpp"
 #include "mp_group.hpp"
 
@@ -26,7 +27,11 @@ struct Cfg {
   static constexpr int kGroups = kBlock / L;
   static constexpr int kLdsWords = NL + kGroups * 2 * S;
   // registers of the Montgomery working set: 2K (accumulators) + K (b) + K (m)
-  static constexpr int kWaves = 4 * K + 40 <= 128 ? 4 : (4 * K + 40 <= 168 ? 3 : 2);
+  static constexpr int kRegWaves = 4 * K + 40 <= 128 ? 4 : (4 * K + 40 <= 168 ? 3 : 2);
+  // a block is one wave per SIMD; LDS (160 KiB per CU) caps resident blocks,
+  // and a register budget tighter than that cap only adds spills
+  static constexpr int kLdsBlocks = (160 * 1024) / (kLdsWords * 4 + 64);
+  static constexpr int kWaves = kRegWaves < kLdsBlocks ? kRegWaves : (kLdsBlocks < 1 ? 1 : kLdsBlocks);
 };
 
 template <int L, int K>
@@ -106,21 +111,28 @@ struct Grp {
   }
 };
 
+// This group's view of the block's LDS (modulus staged at lds[0, NL)).
+template <int L, int K, int W>
+__device__ __forceinline__ Grp<L, K, W> grp_at(u32* lds, const ModArgs& mod) {
+  typedef Cfg<L, K, W> C;
+  Grp<L, K, W> g;
+  g.lid = grp_lane<L>();
+  const int gi = (int)(threadIdx.x / (u32)L);
+  g.A = lds + C::NL + gi * 2 * C::S;
+  g.B = g.A + C::S;
+  g.md.m = lds;
+  g.md.mprime = mod.mprime;
+  return g;
+}
+
 // Block prologue: stage the modulus into LDS, then set up this group.
 template <int L, int K, int W>
 __device__ __forceinline__ Grp<L, K, W> grp_setup(u32* lds, const ModArgs& mod, long& e) {
   typedef Cfg<L, K, W> C;
   for (int i = threadIdx.x; i < C::NL; i += kBlock) lds[i] = mod.m[i];
   __syncthreads();
-  Grp<L, K, W> g;
-  g.lid = grp_lane<L>();
-  const int gi = (int)(threadIdx.x / (u32)L);
-  e = (long)blockIdx.x * C::kGroups + gi;
-  g.A = lds + C::NL + gi * 2 * C::S;
-  g.B = g.A + C::S;
-  g.md.m = lds;
-  g.md.mprime = mod.mprime;
-  return g;
+  e = (long)blockIdx.x * C::kGroups + (int)(threadIdx.x / (u32)L);
+  return grp_at<L, K, W>(lds, mod);
 }
 
 #define LDS_DECL(L, K, W) __shared__ u32 lds[Cfg<L, K, W>::kLdsWords]
@@ -213,12 +225,9 @@ __device__ __forceinline__ void fixed_base_pow(Grp<L, K, W>& g, const PubArgs& P
 // Encryption (encryptor.cc:35-49): c = (1 + n*m) * h_s^r mod n^2, Z-space out.
 // ---------------------------------------------------------------------------
 template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_i64(PubArgs P, const i64* __restrict__ m, const u32* __restrict__ r, int r_words,
-                                    long count, u32* __restrict__ out, u32* err) {
-  LDS_DECL(L, K, W);
-  long e;
-  auto g = grp_setup<L, K, W>(lds, P.n2, e);
-  if (e >= count) return;
+__device__ __forceinline__ void enc_i64_body(Grp<L, K, W>& g, const PubArgs& P, const i64* __restrict__ m,
+                                             const u32* __restrict__ r, int r_words, long e, u32* __restrict__ out,
+                                             u32* err) {
   const u32* rp = r + e * r_words;
   if (!r_fits(rp, r_words, P.windows * P.unit)) {
     if (g.lid == 0) atomicOr(err, kErrRange);
@@ -229,6 +238,17 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_i64(
   gm_from_i64<L, K>(gm, m[e], P.n29, g.md, g.lid);
   g.mul(gm);  // (h_s^r R) * gm * R^-1 = h_s^r * gm
   g.store_canonical(out + e * W, W);
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_i64(PubArgs P, const i64* __restrict__ m,
+                                                                              const u32* __restrict__ r, int r_words,
+                                                                              long count, u32* __restrict__ out,
+                                                                              u32* err) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, P.n2, e);
+  if (e < count) enc_i64_body(g, P, m, r, r_words, e, out, err);
 }
 
 // Encryption of plaintexts given as two's complement over mw words (|m| <= n/2
@@ -289,12 +309,9 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_big(
 // Randomize (evaluator.cc:25-28) and EncryptZero (encryptor.cc:33):
 // out = c * h_s^r mod n^2 (c = 1 when ct is null).
 template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_randomize(PubArgs P, const u32* __restrict__ ct, const u32* __restrict__ r, int r_words,
-                                  long count, u32* __restrict__ out, u32* err) {
-  LDS_DECL(L, K, W);
-  long e;
-  auto g = grp_setup<L, K, W>(lds, P.n2, e);
-  if (e >= count) return;
+__device__ __forceinline__ void randomize_body(Grp<L, K, W>& g, const PubArgs& P, const u32* __restrict__ ct,
+                                               const u32* __restrict__ r, int r_words, long e, u32* __restrict__ out,
+                                               u32* err) {
   const u32* rp = r + e * r_words;
   if (!r_fits(rp, r_words, P.windows * P.unit)) {
     if (g.lid == 0) atomicOr(err, kErrRange);
@@ -309,17 +326,24 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_randomize(Pu
   g.store_canonical(out + e * W, W);
 }
 
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_randomize(PubArgs P, const u32* __restrict__ ct,
+                                                                            const u32* __restrict__ r, int r_words,
+                                                                            long count, u32* __restrict__ out,
+                                                                            u32* err) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, P.n2, e);
+  if (e < count) randomize_body(g, P, ct, r, r_words, e, out, err);
+}
+
 // ---------------------------------------------------------------------------
 // ct (+) ct = a*b mod n^2 (evaluator.cc:30-44) and ct (+) pt = c*(1+n*p)
 // (evaluator.cc:46-64).  Z-space in and out: (a*b*R^-1) * R^2 * R^-1.
 // ---------------------------------------------------------------------------
 template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mulmod(ModArgs mod, const u32* __restrict__ a, const u32* __restrict__ b, long count,
-                               u32* __restrict__ out) {
-  LDS_DECL(L, K, W);
-  long e;
-  auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e >= count) return;
+__device__ __forceinline__ void mulmod_body(Grp<L, K, W>& g, const ModArgs& mod, const u32* __restrict__ a,
+                                            const u32* __restrict__ b, long e, u32* __restrict__ out) {
   u32 x[K];
   g.load_words(x, a + e * W, W);
   g.set(g.A, x);
@@ -330,12 +354,18 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mulmod(ModAr
 }
 
 template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_add_pt_i64(PubArgs P, const u32* __restrict__ ct, const i64* __restrict__ pt, long count,
-                                   u32* __restrict__ out) {
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mulmod(ModArgs mod, const u32* __restrict__ a,
+                                                                         const u32* __restrict__ b, long count,
+                                                                         u32* __restrict__ out) {
   LDS_DECL(L, K, W);
   long e;
-  auto g = grp_setup<L, K, W>(lds, P.n2, e);
-  if (e >= count) return;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e < count) mulmod_body(g, mod, a, b, e, out);
+}
+
+template <int L, int K, int W>
+__device__ __forceinline__ void add_pt_body(Grp<L, K, W>& g, const PubArgs& P, const u32* __restrict__ ct,
+                                            const i64* __restrict__ pt, long e, u32* __restrict__ out) {
   u32 x[K];
   g.load_words(x, ct + e * W, W);
   g.set(g.A, x);
@@ -343,6 +373,16 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_add_pt_i64(P
   g.mul(x);
   g.mul_global(P.n2.r2);
   g.store_canonical(out + e * W, W);
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_add_pt_i64(PubArgs P, const u32* __restrict__ ct,
+                                                                             const i64* __restrict__ pt, long count,
+                                                                             u32* __restrict__ out) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, P.n2, e);
+  if (e < count) add_pt_body(g, P, ct, pt, e, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -378,13 +418,9 @@ __device__ __forceinline__ void pow_u64(Grp<L, K, W>& g, const ModArgs& mod, u64
 // returns c unchanged; for k < 0 this writes c^|k| and sets neg[e] so the
 // engine inverts it afterwards.
 template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_pow_i64(ModArgs mod, const u32* __restrict__ ct, const i64* __restrict__ k, long count,
-                                u32* __restrict__ out, u8* neg, u32* tab) {
-  constexpr int NL = L * K;
-  LDS_DECL(L, K, W);
-  long e;
-  auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e >= count) return;
+__device__ __forceinline__ void pow_i64_body(Grp<L, K, W>& g, const ModArgs& mod, const u32* __restrict__ ct,
+                                             const i64* __restrict__ k, long e, u32* __restrict__ out, u8* neg,
+                                             u32* __restrict__ T) {
   const i64 kk = k[e];
   if (neg && g.lid == 0) neg[e] = kk < 0 ? 1 : 0;
   if (kk == 0 || kk == 1) {
@@ -396,9 +432,20 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_pow_i64(ModA
   g.load_words(x, ct + e * W, W);
   g.set(g.A, x);
   g.mul_global(mod.r2);  // c R
-  pow_u64(g, mod, mag, tab + e * (long)(NL << kPowWin));
+  pow_u64(g, mod, mag, T);
   g.from_mont();
   g.store_canonical(out + e * W, W);
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_pow_i64(ModArgs mod, const u32* __restrict__ ct,
+                                                                          const i64* __restrict__ k, long count,
+                                                                          u32* __restrict__ out, u8* neg, u32* tab) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e < count) pow_i64_body(g, mod, ct, k, e, out, neg, tab + e * (long)(NL << kPowWin));
 }
 
 // ct (x) scalar for exponents of any length: |k| as kw little-endian words
@@ -461,21 +508,13 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_pow_words(Mo
 // so control flow is uniform.
 // ---------------------------------------------------------------------------
 template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_dec_half(DecArgs D, const u32* __restrict__ ct, long count, u32* __restrict__ halves,
-                                 u32* tab) {
+__device__ __forceinline__ void dec_half_body(Grp<L, K, W>& g, const ModArgs& mod, const u32* __restrict__ sched,
+                                              int nsched, const u32* __restrict__ c, u32* __restrict__ T,
+                                              u32* __restrict__ dst) {
   constexpr int NL = L * K;
-  constexpr int HW = W / 2;
-  LDS_DECL(L, K, W);
-  const int h = blockIdx.y;
-  const ModArgs& mod = h ? D.q2 : D.p2;
-  long e;
-  auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e >= count) return;
-  const u32* sched = h ? D.sched_q : D.sched_p;
-  const int nsched = h ? D.nsched_q : D.nsched_p;
   {
     u32 lo[K], hi[K];
-    stage_words<L>(g.A, ct + e * W, W);  // spans A..B
+    stage_words<L>(g.A, c, W);  // spans A..B
     words_to_limbs<L, K>(g.A, W, 0, lo);
     words_to_limbs<L, K>(g.A, W, NL, hi);
     g.set(g.A, hi);
@@ -493,7 +532,6 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_dec_half(Dec
     seminorm<L, K>(s, x);  // c R mod p^2 (lazy, < 3 p^2)
     g.set(g.A, x);
   }
-  u32* T = tab + (h * count + e) * (long)(NL << (kDecWin - 1));
   g.save(T, g.A);
   {
     u32 b[K], x2[K];
@@ -515,7 +553,23 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_dec_half(Dec
     if (idx) g.mul_global(T + (idx - 1) * NL);
   }
   g.from_mont();
-  g.store_canonical(halves + (h * count + e) * HW, HW);
+  g.store_canonical(dst, W / 2);
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_dec_half(DecArgs D, const u32* __restrict__ ct,
+                                                                           long count, u32* __restrict__ halves,
+                                                                           u32* tab) {
+  constexpr int NL = L * K;
+  constexpr int HW = W / 2;
+  LDS_DECL(L, K, W);
+  const int h = blockIdx.y;
+  const ModArgs& mod = h ? D.q2 : D.p2;
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  dec_half_body(g, mod, h ? D.sched_q : D.sched_p, h ? D.nsched_q : D.nsched_p, ct + e * W,
+                tab + (h * count + e) * (long)(NL << (kDecWin - 1)), halves + (h * count + e) * HW);
 }
 
 // ---------------------------------------------------------------------------
@@ -683,18 +737,13 @@ __global__ __launch_bounds__(kBlock) void k_dec_combine(CombArgs C, const u32* _
 // Partials are R29 interleaved.
 // ---------------------------------------------------------------------------
 template <int L, int K, int W, bool ZIN>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_sum(ModArgs mod, const u32* __restrict__ in, const Chunk* __restrict__ chunks,
-                                   long nchunks, u32* __restrict__ part) {
+__device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const u32* __restrict__ in, const Chunk ch,
+                                                u32* __restrict__ part) {
   constexpr int NL = L * K;
-  LDS_DECL(L, K, W);
-  long e;
-  auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e >= nchunks) return;
-  const Chunk ch = chunks[e];
   u32 x[K];
   if (ch.len == 0) {
     one_limbs<L, K>(x, g.lid);
-    store_r29<L, K>(part + e * NL, x, g.lid);
+    store_r29<L, K>(part, x, g.lid);
     return;
   }
   if constexpr (ZIN) g.load_words(x, in + ch.start * W, W);
@@ -705,28 +754,30 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_sum(M
     else load_r29<L, K>(x, in + i * NL, g.lid);
     g.mul(x);
   }
-  g.save(part + e * NL, g.A);
+  g.save(part, g.A);
 }
 
-template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_dot0(ModArgs mod, const u32* __restrict__ in, const i64* __restrict__ wts,
-                                    const Chunk* __restrict__ chunks, long nchunks, u32* __restrict__ part_pos,
-                                    u32* __restrict__ part_neg, u32* tab) {
+template <int L, int K, int W, bool ZIN>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_sum(ModArgs mod, const u32* __restrict__ in,
+                                                                             const Chunk* __restrict__ chunks,
+                                                                             long nchunks, u32* __restrict__ part) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e >= nchunks) return;
-  const Chunk ch = chunks[e];
-  u32* pp = part_pos + e * NL;
-  u32* pn = part_neg + e * NL;
+  if (e < nchunks) reduce_sum_body<L, K, W, ZIN>(g, in, chunks[e], part + e * NL);
+}
+
+template <int L, int K, int W>
+__device__ __forceinline__ void reduce_dot0_body(Grp<L, K, W>& g, const ModArgs& mod, const u32* __restrict__ in,
+                                                 const i64* __restrict__ wts, const Chunk ch, u32* __restrict__ pp,
+                                                 u32* __restrict__ pn, u32* __restrict__ T) {
   {
     u32 one[K];
     load_r29<L, K>(one, mod.one, g.lid);
     store_r29<L, K>(pp, one, g.lid);
     store_r29<L, K>(pn, one, g.lid);
   }
-  u32* T = tab + e * (long)(NL << kPowWin);
   for (long i = ch.start; i < ch.start + ch.len; ++i) {
     const i64 w = wts[i];
     if (w == 0) continue;
@@ -743,6 +794,21 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_dot0(
     g.mul(x);
     g.save(acc, g.A);
   }
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_dot0(ModArgs mod, const u32* __restrict__ in,
+                                                                              const i64* __restrict__ wts,
+                                                                              const Chunk* __restrict__ chunks,
+                                                                              long nchunks, u32* __restrict__ part_pos,
+                                                                              u32* __restrict__ part_neg, u32* tab) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e < nchunks)
+    reduce_dot0_body(g, mod, in, wts, chunks[e], part_pos + e * NL, part_neg + e * NL,
+                     tab + e * (long)(NL << kPowWin));
 }
 
 // Upper levels of a dot: products of Montgomery-form partials (stay in M-space).
@@ -1121,6 +1187,96 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mul_const(Mo
 }
 
 // ---------------------------------------------------------------------------
+// Persistent work-list kernel.  Workgroups pull units (blocks of kGroups
+// elements of one item) from ctr[0] until the list is exhausted, so one launch
+// covers vectors of any length and any key of this shape, and long and short
+// items balance across the chip.  Window tables are per group slot
+// (blockIdx, group), reused from unit to unit.  kDec selects the decryption
+// half kernel (moduli p^2 / q^2); otherwise the n^2 ops.  The last workgroup to
+// finish clears the counters, so launches need no memset between them (and a
+// captured graph can replay the launch as is).
+// ---------------------------------------------------------------------------
+template <int L, int K, int W, bool kDec>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_worklist(const WorkItem* __restrict__ items,
+                                                                           int nitems, u32 total_units, u32* ctr,
+                                                                           u32* slot_tab) {
+  typedef Cfg<L, K, W> C;
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  __shared__ u32 s_unit;
+  const int gi = (int)(threadIdx.x / (u32)L);
+  const long slot = (long)blockIdx.x * C::kGroups + gi;
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = atomicAdd(ctr, 1u);
+    __syncthreads();
+    // uniform by construction; readfirstlane lets the item lookup and every
+    // item field live in scalar registers
+    const u32 u = __builtin_amdgcn_readfirstlane(s_unit);
+    if (u >= total_units) break;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].unit0 <= (long)u) lo = mid;
+      else hi = mid - 1;
+    }
+    const WorkItem& it = items[lo];
+    for (int i = threadIdx.x; i < NL; i += kBlock) lds[i] = it.pub.n2.m[i];
+    __syncthreads();
+    auto g = grp_at<L, K, W>(lds, it.pub.n2);
+    const long e = ((long)u - it.unit0) * C::kGroups + gi;
+    if (e < it.count) {
+      if constexpr (kDec) {
+        dec_half_body(g, it.pub.n2, static_cast<const u32*>(it.c), it.aux, static_cast<const u32*>(it.a) + e * W,
+                      slot_tab + slot * (long)(NL << (kDecWin - 1)), static_cast<u32*>(it.out) + e * (W / 2));
+      } else {
+        u32* T = slot_tab + slot * (long)(NL << kPowWin);
+        switch (it.op) {
+          case kWorkEncrypt:
+            enc_i64_body(g, it.pub, static_cast<const i64*>(it.a), static_cast<const u32*>(it.b), it.aux, e,
+                         static_cast<u32*>(it.out), it.err);
+            break;
+          case kWorkRandomize:
+            randomize_body(g, it.pub, static_cast<const u32*>(it.a), static_cast<const u32*>(it.b), it.aux, e,
+                           static_cast<u32*>(it.out), it.err);
+            break;
+          case kWorkAdd:
+            mulmod_body(g, it.pub.n2, static_cast<const u32*>(it.a), static_cast<const u32*>(it.b), e,
+                        static_cast<u32*>(it.out));
+            break;
+          case kWorkAddPt:
+            add_pt_body(g, it.pub, static_cast<const u32*>(it.a), static_cast<const i64*>(it.b), e,
+                        static_cast<u32*>(it.out));
+            break;
+          case kWorkMul:
+            pow_i64_body(g, it.pub.n2, static_cast<const u32*>(it.a), static_cast<const i64*>(it.b), e,
+                         static_cast<u32*>(it.out), it.neg, T);
+            break;
+          case kWorkDot0:
+            reduce_dot0_body(g, it.pub.n2, static_cast<const u32*>(it.a), static_cast<const i64*>(it.b),
+                             static_cast<const Chunk*>(it.c)[e], static_cast<u32*>(it.out) + e * NL,
+                             static_cast<u32*>(it.out2) + e * NL, T);
+            break;
+          case kWorkSum0:
+            reduce_sum_body<L, K, W, true>(g, static_cast<const u32*>(it.a), static_cast<const Chunk*>(it.c)[e],
+                                           static_cast<u32*>(it.out) + e * NL);
+            break;
+          default:
+            break;
+        }
+      }
+    }
+    __syncthreads();  // LDS (modulus, slots, s_unit) is rewritten by the next unit
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(ctr + 1, 1u) == gridDim.x - 1) {
+      atomicExch(ctr, 0u);
+      atomicExch(ctr + 1, 0u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers, one instantiation per supported shape
 // ---------------------------------------------------------------------------
 template <int L, int K, int W>
@@ -1215,6 +1371,28 @@ struct Launch {
     if (n > 0)
       hipLaunchKernelGGL((k_dec_half<L, K, W>), dim3(blocks_for(n, L), 2), dim3(kBlock), 0, st, D, ct, n, halves, tab);
   }
+  template <bool kDec>
+  static void worklist(hipStream_t st, const WorkItem* items, int nitems, u32 total_units, u32* ctr, u32* slot_tab,
+                       unsigned grid) {
+    if (total_units > 0 && grid > 0)
+      hipLaunchKernelGGL((k_worklist<L, K, W, kDec>), dim3(grid), dim3(kBlock), 0, st, items, nitems, total_units, ctr,
+                         slot_tab);
+  }
+  template <bool kDec>
+  static int resident() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_worklist<L, K, W, kDec>, kBlock, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+  }
+  template <bool kDec>
+  static WorklistShape worklist_shape() {
+    WorklistShape w;
+    w.launch = worklist<kDec>;
+    w.resident = resident<kDec>;
+    w.groups_per_unit = kBlock / L;
+    w.slot_table_words = (long)(L * K) << (kDec ? kDecWin - 1 : kPowWin);
+    return w;
+  }
 };
 
 template <int PW>
@@ -1233,8 +1411,10 @@ typedef Shape<8, 27, 192> ShapeN3072;
 typedef Shape<2, 36, 128> ShapeP2048;
 typedef Shape<4, 27, 192> ShapeP3072;
 
+// The launch table of one key size; the work-list kernels come from their own
+// translation units (they are the largest kernels to compile).
 template <class SN, class SP, int PW>
-static const Ops make_ops() {
+static Ops make_ops(const WorklistShape& wl_n, const WorklistShape& wl_p) {
   Ops o;
   o.n_L = SN::kL;
   o.n_K = SN::kK;
@@ -1261,15 +1441,9 @@ static const Ops make_ops() {
   o.mul_const = Launch<SN>::mul_const;
   o.dec_half = Launch<SP>::dec_half;
   o.combine = launch_combine<PW>;
+  o.wl_n = wl_n;
+  o.wl_p = wl_p;
   return o;
-}
-
-const Ops* ops_for_key_bits(int key_bits) {
-  static const Ops k2048 = make_ops<ShapeN2048, ShapeP2048, 32>();
-  static const Ops k3072 = make_ops<ShapeN3072, ShapeP3072, 48>();
-  if (key_bits == 2048) return &k2048;
-  if (key_bits == 3072) return &k3072;
-  return nullptr;
 }
 
 }  // namespace heu_hip

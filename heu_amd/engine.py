@@ -347,5 +347,116 @@ class Key:
         return out
 
 
+class Batch:
+    """Many tasks (vectors of any length, on any mix of 2048/3072-bit keys) run
+    as one dispatch: persistent work-list launches plus follow-up steps,
+    optionally captured into a hipGraph and replayed (heu_batch_* C ABI).
+
+    Each method records a task and returns its (preallocated) output tensor;
+    `run()` plans the batch on first use and enqueues one run.  Inputs are kept
+    alive by the batch; the same buffers are read at every run."""
+
+    def __init__(self, ctx: Context, graph: bool = False):
+        self.ctx = ctx
+        self.graph = graph
+        self._tasks = []
+        self._keep = []
+        self._h = None
+
+    def _task(self, key, op, a=None, b=None, count=0, out=None, aux=0, offsets=None, segments=0):
+        if self._h is not None:
+            raise RuntimeError("batch already planned")
+        self._keep += [x for x in (a, b, out, offsets) if x is not None]
+        addr = lambda x: None if x is None else (x.data_ptr() if isinstance(x, torch.Tensor) else x.ctypes.data)
+        t = _native.HeuTask(key._h.value, op, aux, addr(a), addr(b), count, addr(offsets), segments, addr(out))
+        self._tasks.append(t)
+        return out
+
+    def encrypt_i64(self, key: "Key", m, r, out: torch.Tensor | None = None) -> torch.Tensor:
+        m, r = key._dev(m, torch.int64), key._dev(r, torch.int32)
+        n = m.shape[0]
+        if out is None:
+            out = key._ct_out(n)
+        elif not out.is_contiguous() or tuple(out.shape) != (n, key.ct_words) or out.dtype != torch.int32:
+            raise ValueError(f"out must be a contiguous int32 tensor of shape ({n}, {key.ct_words})")
+        return self._task(key, _native.HEU_OP_ENCRYPT_I64, m, r, n, out, aux=r.shape[1])
+
+    def randomize(self, key: "Key", ct, r) -> torch.Tensor:
+        r = key._dev(r, torch.int32)
+        ct = None if ct is None else key._dev(ct, torch.int32)
+        n = r.shape[0]
+        return self._task(key, _native.HEU_OP_RANDOMIZE, ct, r, n, key._ct_out(n), aux=r.shape[1])
+
+    def add(self, key: "Key", a, b) -> torch.Tensor:
+        a, b = key._dev(a, torch.int32), key._dev(b, torch.int32)
+        n = key._rows(a)
+        return self._task(key, _native.HEU_OP_ADD, a, b, n, key._ct_out(n))
+
+    def add_i64(self, key: "Key", a, p) -> torch.Tensor:
+        a, p = key._dev(a, torch.int32), key._dev(p, torch.int64)
+        n = key._rows(a)
+        return self._task(key, _native.HEU_OP_ADD_I64, a, p, n, key._ct_out(n))
+
+    def mul_i64(self, key: "Key", a, k) -> torch.Tensor:
+        a, k = key._dev(a, torch.int32), key._dev(k, torch.int64)
+        n = key._rows(a)
+        return self._task(key, _native.HEU_OP_MUL_I64, a, k, n, key._ct_out(n))
+
+    def decrypt(self, key: "Key", ct, out_words: int | None = None) -> torch.Tensor:
+        ct = key._dev(ct, torch.int32)
+        n = key._rows(ct)
+        ow = out_words or key.n_words
+        out = torch.empty((n, ow), dtype=torch.int32, device=key.ctx.device)
+        return self._task(key, _native.HEU_OP_DECRYPT, ct, None, n, out, aux=ow)
+
+    def sum(self, key: "Key", ct, offsets) -> torch.Tensor:
+        ct = key._dev(ct, torch.int32)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        segs = len(offs) - 1
+        return self._task(key, _native.HEU_OP_SUM, ct, None, 0, key._ct_out(segs), offsets=offs, segments=segs)
+
+    def dot_i64(self, key: "Key", ct, w, offsets) -> torch.Tensor:
+        ct, w = key._dev(ct, torch.int32), key._dev(w, torch.int64)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        segs = len(offs) - 1
+        return self._task(key, _native.HEU_OP_DOT_I64, ct, w, 0, key._ct_out(segs), offsets=offs, segments=segs)
+
+    def barrier(self) -> None:
+        """Tasks recorded after this see the outputs of every task before it."""
+        if self._h is not None:
+            raise RuntimeError("batch already planned")
+        self._tasks.append(_native.HeuTask(None, _native.HEU_OP_BARRIER, 0, None, None, 0, None, 0, None))
+
+    def plan(self) -> None:
+        if self._h is not None:
+            return
+        arr = (_native.HeuTask * len(self._tasks))(*self._tasks)
+        h = ctypes.c_void_p()
+        flags = _native.HEU_BATCH_GRAPH if self.graph else 0
+        check(lib().heu_batch_create(self.ctx._h, arr, len(self._tasks), flags, ctypes.byref(h)))
+        self._h = h
+
+    def run(self) -> None:
+        self.plan()
+        check(lib().heu_batch_run(self._h))
+
+    def stats(self) -> dict:
+        self.plan()
+        la, un, by = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_size_t()
+        check(lib().heu_batch_stats(self._h, ctypes.byref(la), ctypes.byref(un), ctypes.byref(by)))
+        return {"launches": la.value, "units": un.value, "device_bytes": by.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and _native._lib is not None:
+            lib().heu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def to_host_words(t: torch.Tensor) -> np.ndarray:
     return t.cpu().numpy().view(np.uint32)

@@ -134,6 +134,67 @@ int heu_dot_i64(heu_key *key, const uint32_t *ct, const int64_t *w, const int64_
 int heu_to_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
 int heu_from_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
 
+/* ---- multi-task batches ------------------------------------------------ */
+/* A batch runs many tasks -- vectors of any length, on any mix of resident
+ * 2048- and 3072-bit keys -- as one dispatch.  The elementwise work of every
+ * task runs in persistent work-list launches, one per (key size, modulus
+ * shape): workgroups pull units of elements from a shared counter, so short
+ * and long vectors balance across the chip.  Dependent follow-up work (CRT
+ * recombination, batch inversion for negative scalars, upper reduction levels)
+ * follows in task order.  Tasks between barriers (HEU_OP_BARRIER) are
+ * independent: none may read another's output.  With HEU_BATCH_GRAPH the whole sequence is captured
+ * once into a hipGraph and every run replays it (one hipGraphLaunch).
+ *
+ * This is the shape HEU's numpy layer hands a vectorized backend -- one span
+ * per yacl worker chunk, numpy/encryptor.cc:42-52, numpy/evaluator.cc:120-186
+ * -- gathered across operations and keys; it has no single counterpart in the
+ * reference, whose GPU backend makes one synchronous call per op
+ * (paillier_gpu/gpulib/gpupaillier.h:17-91). */
+typedef struct heu_batch heu_batch;
+
+enum {
+  HEU_OP_ENCRYPT_I64 = 1, /* a: int64 m[count], b: r (aux limbs each); out: ct */
+  HEU_OP_RANDOMIZE = 2,   /* a: ct or NULL (EncryptZero), b: r (aux limbs each); out: ct */
+  HEU_OP_ADD = 3,         /* a, b: ct; out: a*b */
+  HEU_OP_ADD_I64 = 4,     /* a: ct, b: int64 pt; out: ct*(1+n*pt) */
+  HEU_OP_MUL_I64 = 5,     /* a: ct, b: int64 scalar; out: ct^k (inverse power for k < 0) */
+  HEU_OP_DECRYPT = 6,     /* a: ct; out: plaintext, aux limbs two's complement each */
+  HEU_OP_SUM = 7,         /* a: ct; offsets/segments; out: one ct per segment */
+  HEU_OP_DOT_I64 = 8,     /* a: ct, b: int64 weights; offsets/segments; out: one ct per segment */
+  HEU_OP_BARRIER = 9      /* no operands: tasks after it see the outputs of every task before it */
+};
+
+typedef struct heu_task {
+  heu_key *key;
+  int32_t op;              /* HEU_OP_* */
+  uint32_t aux;            /* r limbs (ENCRYPT_I64, RANDOMIZE) or output limbs (DECRYPT) */
+  const void *a;           /* device inputs as listed per op */
+  const void *b;
+  size_t count;            /* elements (for SUM / DOT: offsets[segments] - offsets[0] is ignored) */
+  const int64_t *offsets;  /* SUM / DOT: HOST array of segments + 1 element offsets into a */
+  size_t segments;
+  void *out;               /* device output */
+} heu_task;
+
+#define HEU_BATCH_GRAPH 1u /* capture into a hipGraph at the first run, replay afterwards */
+
+/* Plans a batch: sizes and allocates its own device workspace, uploads its
+ * work lists, and copies `tasks` (offsets included).  Device pointers in the
+ * tasks are used as given at every run, and every key must outlive the batch. */
+int heu_batch_create(heu_ctx *ctx, const heu_task *tasks, size_t ntasks, uint32_t flags, heu_batch **out);
+/* Enqueues one run on the context's stream (asynchronous; errors found by
+ * kernels surface at heu_ctx_sync). */
+int heu_batch_run(heu_batch *batch);
+int heu_batch_destroy(heu_batch *batch);
+/* Kernel launches per run (graph nodes when captured), work units in the
+ * persistent launches, and device bytes held by the batch. */
+int heu_batch_stats(const heu_batch *batch, uint64_t *launches, uint64_t *units, size_t *device_bytes);
+
+/* Persistent launch width: workgroups = multiplier x resident workgroups per
+ * CU x CUs (multiplier 0 = one workgroup per unit, i.e. not persistent).
+ * Applies to batches planned afterwards; the autotuner sets it per shape. */
+int heu_ctx_set_grid_multiplier(heu_ctx *ctx, uint32_t key_bits, int decrypt_shape, uint32_t multiplier);
+
 #ifdef __cplusplus
 }
 #endif
