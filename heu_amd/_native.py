@@ -49,6 +49,11 @@ SIGNATURES = {
     "heu_batch_destroy": (I, [P]),
     "heu_batch_stats": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(SZ)]),
     "heu_ctx_set_grid_multiplier": (I, [P, U32, I, U32]),
+    "heu_server_create": (I, [P, P, ctypes.POINTER(P)]),
+    "heu_server_submit": (I, [P, P, ctypes.POINTER(U64)]),
+    "heu_server_wait": (I, [P, U64, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
+    "heu_server_stats": (I, [P, P]),
+    "heu_server_destroy": (I, [P]),
 }
 
 # heu_batch ops and the heu_task struct (include/heu_paillier.h)
@@ -94,3 +99,13 @@ def check(rc: int) -> None:
     if rc != HEU_OK:
         msg = lib().heu_last_error()
         raise HeuError(rc, msg.decode() if msg else "")
+
+
+class HeuServerConfig(ctypes.Structure):
+    _fields_ = [("workspace_bytes", SZ), ("max_inflight", U32), ("max_batch_tasks", U32),
+                ("max_batch_units", U64)]
+
+
+class HeuServerStats(ctypes.Structure):
+    _fields_ = [("submitted", U64), ("completed", U64), ("failed", U64), ("batches", U64),
+                ("max_batch_requests", U64), ("peak_workspace_bytes", SZ)]

@@ -389,35 +389,51 @@ int plan(Builder& b, heu_ctx* ctx, const std::vector<heu_task>& tasks, uint64_t*
 
 }  // namespace
 
+namespace heu_engine {
+
+int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, PlanOut* out) {
+  std::vector<heu_task> ts(tasks, tasks + ntasks);
+  for (size_t i = 0; i < ntasks; ++i) {
+    int rc = check_task(ctx, ts[i], i);
+    if (rc) return rc;
+  }
+  Builder b;
+  b.base = base;
+  uint64_t units = 0;
+  int rc = plan(b, ctx, ts, &units);
+  if (rc) return rc;
+  out->bytes = b.used;
+  out->launches = b.launches;
+  out->units = units;
+  out->metas = std::move(b.metas);
+  out->steps = std::move(b.steps);
+  return HEU_OK;
+}
+
+}  // namespace heu_engine
+
 extern "C" {
 
 int heu_batch_create(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, uint32_t flags, heu_batch** out) {
   if (!ctx || !out || (!tasks && ntasks)) return fail(HEU_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
-  std::vector<heu_task> ts(tasks, tasks + ntasks);
-  for (size_t i = 0; i < ntasks; ++i) {
-    int rc = check_task(ctx, ts[i], i);
-    if (rc) return rc;
-  }
   auto batch = std::make_unique<heu_batch>();
   batch->ctx = ctx;
   batch->flags = flags;
-  uint64_t units = 0;
-  Builder sizing;
-  int rc = plan(sizing, ctx, ts, &units);
+  PlanOut sizing, p;
+  int rc = plan_batch(ctx, tasks, ntasks, nullptr, &sizing);
   if (rc) return rc;
-  if ((rc = batch->ws.alloc(std::max<size_t>(sizing.used, 256)))) return rc;
-  Builder b;
-  b.base = batch->ws.as<char>();
-  if ((rc = plan(b, ctx, ts, &units))) return rc;
-  if (b.used != sizing.used) return fail(HEU_EINVAL, "internal: batch plan is not deterministic");
-  for (auto& m : b.metas)
-    HIP_TRY(hipMemcpyAsync(b.base + m.first, m.second.data(), m.second.size(), hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));  // the host images die with `b`
-  batch->steps = std::move(b.steps);
-  batch->launches = b.launches;
-  batch->units = units;
+  if ((rc = batch->ws.alloc(std::max<size_t>(sizing.bytes, 256)))) return rc;
+  if ((rc = plan_batch(ctx, tasks, ntasks, batch->ws.as<char>(), &p))) return rc;
+  if (p.bytes != sizing.bytes) return fail(HEU_EINVAL, "internal: batch plan is not deterministic");
+  for (auto& m : p.metas)
+    HIP_TRY(hipMemcpyAsync(batch->ws.as<char>() + m.first, m.second.data(), m.second.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));  // the host images die with `p`
+  batch->steps = std::move(p.steps);
+  batch->launches = p.launches;
+  batch->units = p.units;
   *out = batch.release();
   return HEU_OK;
 }

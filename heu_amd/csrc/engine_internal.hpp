@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -61,6 +62,19 @@ std::vector<long> inv_levels(long count);
 // Chunking of segments for a reduction level.
 std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, const std::vector<long>& seg_len, long chunk,
                                         std::vector<long>* seg_first_chunk, std::vector<long>* seg_nchunks);
+
+// A planned batch (batch.cpp): with base == nullptr only `bytes` is computed
+// (the sizing pass); otherwise every device pointer lies in [base, base +
+// bytes), `metas` are host images to copy to base + offset before the first
+// run, and `steps` enqueue one run on a stream.
+struct PlanOut {
+  size_t bytes = 0;
+  uint64_t launches = 0, units = 0;
+  std::vector<std::pair<size_t, std::vector<char>>> metas;
+  std::vector<std::function<void(hipStream_t)>> steps;
+};
+// Checks the tasks and plans them (caller holds ctx->mu).
+int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, PlanOut* out);
 
 }  // namespace heu_engine
 

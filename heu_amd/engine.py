@@ -347,30 +347,16 @@ class Key:
         return out
 
 
-class Batch:
-    """Many tasks (vectors of any length, on any mix of 2048/3072-bit keys) run
-    as one dispatch: persistent work-list launches plus follow-up steps,
-    optionally captured into a hipGraph and replayed (heu_batch_* C ABI).
+class _TaskOps:
+    """Task-recording methods shared by Batch and Server: each converts its
+    operands to device tensors, allocates the output and hands one heu_task to
+    `_task`, returning what `_task` returns."""
 
-    Each method records a task and returns its (preallocated) output tensor;
-    `run()` plans the batch on first use and enqueues one run.  Inputs are kept
-    alive by the batch; the same buffers are read at every run."""
-
-    def __init__(self, ctx: Context, graph: bool = False):
-        self.ctx = ctx
-        self.graph = graph
-        self._tasks = []
-        self._keep = []
-        self._h = None
-
-    def _task(self, key, op, a=None, b=None, count=0, out=None, aux=0, offsets=None, segments=0):
-        if self._h is not None:
-            raise RuntimeError("batch already planned")
-        self._keep += [x for x in (a, b, out, offsets) if x is not None]
+    def _record(self, key, op, a=None, b=None, count=0, out=None, aux=0, offsets=None, segments=0):
+        keep = [x for x in (a, b, out, offsets) if x is not None]
         addr = lambda x: None if x is None else (x.data_ptr() if isinstance(x, torch.Tensor) else x.ctypes.data)
         t = _native.HeuTask(key._h.value, op, aux, addr(a), addr(b), count, addr(offsets), segments, addr(out))
-        self._tasks.append(t)
-        return out
+        return self._task(t, out, keep)
 
     def encrypt_i64(self, key: "Key", m, r, out: torch.Tensor | None = None) -> torch.Tensor:
         m, r = key._dev(m, torch.int64), key._dev(r, torch.int32)
@@ -379,47 +365,72 @@ class Batch:
             out = key._ct_out(n)
         elif not out.is_contiguous() or tuple(out.shape) != (n, key.ct_words) or out.dtype != torch.int32:
             raise ValueError(f"out must be a contiguous int32 tensor of shape ({n}, {key.ct_words})")
-        return self._task(key, _native.HEU_OP_ENCRYPT_I64, m, r, n, out, aux=r.shape[1])
+        return self._record(key, _native.HEU_OP_ENCRYPT_I64, m, r, n, out, aux=r.shape[1])
 
     def randomize(self, key: "Key", ct, r) -> torch.Tensor:
         r = key._dev(r, torch.int32)
         ct = None if ct is None else key._dev(ct, torch.int32)
         n = r.shape[0]
-        return self._task(key, _native.HEU_OP_RANDOMIZE, ct, r, n, key._ct_out(n), aux=r.shape[1])
+        return self._record(key, _native.HEU_OP_RANDOMIZE, ct, r, n, key._ct_out(n), aux=r.shape[1])
 
     def add(self, key: "Key", a, b) -> torch.Tensor:
         a, b = key._dev(a, torch.int32), key._dev(b, torch.int32)
         n = key._rows(a)
-        return self._task(key, _native.HEU_OP_ADD, a, b, n, key._ct_out(n))
+        return self._record(key, _native.HEU_OP_ADD, a, b, n, key._ct_out(n))
 
     def add_i64(self, key: "Key", a, p) -> torch.Tensor:
         a, p = key._dev(a, torch.int32), key._dev(p, torch.int64)
         n = key._rows(a)
-        return self._task(key, _native.HEU_OP_ADD_I64, a, p, n, key._ct_out(n))
+        return self._record(key, _native.HEU_OP_ADD_I64, a, p, n, key._ct_out(n))
 
     def mul_i64(self, key: "Key", a, k) -> torch.Tensor:
         a, k = key._dev(a, torch.int32), key._dev(k, torch.int64)
         n = key._rows(a)
-        return self._task(key, _native.HEU_OP_MUL_I64, a, k, n, key._ct_out(n))
+        return self._record(key, _native.HEU_OP_MUL_I64, a, k, n, key._ct_out(n))
 
     def decrypt(self, key: "Key", ct, out_words: int | None = None) -> torch.Tensor:
         ct = key._dev(ct, torch.int32)
         n = key._rows(ct)
         ow = out_words or key.n_words
         out = torch.empty((n, ow), dtype=torch.int32, device=key.ctx.device)
-        return self._task(key, _native.HEU_OP_DECRYPT, ct, None, n, out, aux=ow)
+        return self._record(key, _native.HEU_OP_DECRYPT, ct, None, n, out, aux=ow)
 
     def sum(self, key: "Key", ct, offsets) -> torch.Tensor:
         ct = key._dev(ct, torch.int32)
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
         segs = len(offs) - 1
-        return self._task(key, _native.HEU_OP_SUM, ct, None, 0, key._ct_out(segs), offsets=offs, segments=segs)
+        return self._record(key, _native.HEU_OP_SUM, ct, None, 0, key._ct_out(segs), offsets=offs, segments=segs)
 
     def dot_i64(self, key: "Key", ct, w, offsets) -> torch.Tensor:
         ct, w = key._dev(ct, torch.int32), key._dev(w, torch.int64)
         offs = np.ascontiguousarray(offsets, dtype=np.int64)
         segs = len(offs) - 1
-        return self._task(key, _native.HEU_OP_DOT_I64, ct, w, 0, key._ct_out(segs), offsets=offs, segments=segs)
+        return self._record(key, _native.HEU_OP_DOT_I64, ct, w, 0, key._ct_out(segs), offsets=offs, segments=segs)
+
+
+class Batch(_TaskOps):
+    """Many tasks (vectors of any length, on any mix of 2048/3072-bit keys) run
+    as one dispatch: persistent work-list launches plus follow-up steps,
+    optionally captured into a hipGraph and replayed (heu_batch_* C ABI).
+
+    Each method records a task and returns its (preallocated) output tensor;
+    `run()` plans the batch on first use and enqueues one run.  Inputs are kept
+    alive by the batch; the same buffers are read at every run.  Tasks between
+    barriers must not read each other's outputs."""
+
+    def __init__(self, ctx: Context, graph: bool = False):
+        self.ctx = ctx
+        self.graph = graph
+        self._tasks = []
+        self._keep = []
+        self._h = None
+
+    def _task(self, t, out, keep):
+        if self._h is not None:
+            raise RuntimeError("batch already planned")
+        self._keep += keep
+        self._tasks.append(t)
+        return out
 
     def barrier(self) -> None:
         """Tasks recorded after this see the outputs of every task before it."""
@@ -450,6 +461,56 @@ class Batch:
         if getattr(self, "_h", None) is not None and _native._lib is not None:
             lib().heu_batch_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server(_TaskOps):
+    """Request server with continuous batching (heu_server_* C ABI).
+
+    Every op method submits one request and returns (ticket, output tensor);
+    `wait(ticket)` blocks until the output is written and returns the request's
+    latency in ms (submit -> completion).  Inputs must be on the device (they
+    are, once an op method returns: host operands are copied synchronously), and
+    the server keeps every operand alive until the request is waited."""
+
+    def __init__(self, ctx: Context, workspace_bytes: int = 0, max_inflight: int = 0, max_batch_tasks: int = 0,
+                 max_batch_units: int = 0):
+        self.ctx = ctx
+        self._keep = {}
+        cfg = _native.HeuServerConfig(workspace_bytes, max_inflight, max_batch_tasks, max_batch_units)
+        h = ctypes.c_void_p()
+        check(lib().heu_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    def _task(self, t, out, keep):
+        tk = ctypes.c_uint64()
+        check(lib().heu_server_submit(self._h, ctypes.byref(t), ctypes.byref(tk)))
+        self._keep[tk.value] = keep
+        return tk.value, out
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0) -> float:
+        lat = ctypes.c_double()
+        rc = lib().heu_server_wait(self._h, ticket, timeout_ms, ctypes.byref(lat))
+        if rc == _native.HEU_OK or "timeout" not in (lib().heu_last_error() or b"").decode():
+            self._keep.pop(ticket, None)
+        check(rc)
+        return lat.value
+
+    def stats(self) -> dict:
+        st = _native.HeuServerStats()
+        check(lib().heu_server_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and _native._lib is not None:
+            lib().heu_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
 
     def __del__(self):
         try:

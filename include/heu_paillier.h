@@ -195,6 +195,45 @@ int heu_batch_stats(const heu_batch *batch, uint64_t *launches, uint64_t *units,
  * Applies to batches planned afterwards; the autotuner sets it per shape. */
 int heu_ctx_set_grid_multiplier(heu_ctx *ctx, uint32_t key_bits, int decrypt_shape, uint32_t multiplier);
 
+/* ---- request server (continuous batching) ------------------------------ */
+/* A scheduler thread admits queued requests (one heu_task each, device
+ * operands, any resident key) into batches while device workspace is free,
+ * runs up to max_inflight batches at once on streams of its own, and retires
+ * them as they finish: requests that arrive while others run are batched and
+ * started as soon as a stream slot and workspace are free, so their work
+ * overlaps on the device.  A request's result is independent of what it was
+ * batched with.  HEU has no counterpart; it serves the request-loop shape of
+ * the path (keys and tables resident, requests of mixed sizes, keys and ops). */
+typedef struct heu_server heu_server;
+
+typedef struct heu_server_config {
+  size_t workspace_bytes;   /* device workspace for in-flight batches (0: 8 GiB) */
+  uint32_t max_inflight;    /* batches on the device at once (0: 2) */
+  uint32_t max_batch_tasks; /* requests per batch (0: 512) */
+  uint64_t max_batch_units; /* work-list units per batch, bounds batch latency (0: no bound) */
+} heu_server_config;
+
+typedef struct heu_server_counters {
+  uint64_t submitted, completed, failed, batches;
+  uint64_t max_batch_requests; /* most requests seen in one batch */
+  size_t peak_workspace_bytes;
+} heu_server_counters;
+
+int heu_server_create(heu_ctx *ctx, const heu_server_config *cfg, heu_server **out);
+/* Queues one request; *ticket identifies it.  Thread-safe.  The task is
+ * checked here (bad arguments fail the call); keys and buffers must stay
+ * valid until the request completes. */
+int heu_server_submit(heu_server *srv, const heu_task *task, uint64_t *ticket);
+/* Waits for a request (timeout_ms < 0: no limit).  HEU_OK: its output is
+ * written and *latency_ms (may be NULL) is submit -> completion on the host
+ * clock; HEU_EINVAL for an unknown ticket; HEU_EDEVICE on timeout or if the
+ * request could not be run (e.g. it alone exceeds the workspace).  A waited
+ * ticket is forgotten. */
+int heu_server_wait(heu_server *srv, uint64_t ticket, double timeout_ms, double *latency_ms);
+int heu_server_stats(heu_server *srv, heu_server_counters *stats);
+/* Drains every queued and running request, then stops. */
+int heu_server_destroy(heu_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,132 @@
+"""GPU tests of the request server (heu_server_*): requests of mixed ops,
+sizes and keys submitted back to back are co-batched, and every result is
+bit-exact the same as the same request run alone (one-op entry points)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from heu_amd import engine
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return engine
+
+
+@pytest.fixture(scope="module")
+def ctx(eng):
+    return eng.Context(0)
+
+
+@pytest.fixture(scope="module")
+def keys(eng, ctx):
+    out = []
+    for bits, seed in ((2048, 41), (3072, 42), (2048, 43)):
+        kd = eng.keygen(bits, seed)
+        out.append(eng.Key.from_keygen(ctx, kd, r_bits=256, window_bits=10))
+    return out
+
+
+def host(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def make_requests(eng, keys, count, seed):
+    """(op, key, args) with mixed sizes; args are device tensors / arrays."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    ops = ["dec", "dec"] + list(rng.choice(["enc", "dec", "add", "mul", "dot", "sum"], size=count - 2))
+    for i, op in enumerate(ops):
+        key = keys[i % len(keys)]
+        n = int(np.exp(rng.uniform(0, np.log(3000)))) if i >= 2 else 6000
+        m = rng.integers(-(2**62), 2**62, size=n, dtype=np.int64)
+        r = eng.seeded_r(seed, i, n, 256)
+        ct = key.encrypt_i64(m, r)
+        if op == "enc":
+            args = (m, r)
+        elif op == "dec":
+            args = (ct,)
+        elif op == "add":
+            args = (ct, key.encrypt_i64(m[::-1].copy(), r))
+        elif op == "mul":
+            args = (ct, rng.integers(-(2**31), 2**31 - 1, size=n, dtype=np.int64))
+        else:
+            cuts = np.sort(rng.integers(0, n + 1, size=3))
+            offs = np.concatenate([[0], cuts, [n]]).astype(np.int64)
+            w = rng.integers(-(2**31), 2**31 - 1, size=n, dtype=np.int64)
+            args = (ct, w, offs) if op == "dot" else (ct, offs)
+        reqs.append((op, key, args))
+    return reqs
+
+
+def alone(key, op, args):
+    if op == "enc":
+        return key.encrypt_i64(*args)
+    if op == "dec":
+        return key.decrypt(args[0], out_words=3)
+    if op == "add":
+        return key.add(*args)
+    if op == "mul":
+        return key.mul_i64(*args)
+    if op == "dot":
+        return key.dot_i64(*args)
+    return key.sum(*args)
+
+
+def submit(srv, key, op, args):
+    if op == "enc":
+        return srv.encrypt_i64(key, *args)
+    if op == "dec":
+        return srv.decrypt(key, args[0], out_words=3)
+    if op == "add":
+        return srv.add(key, *args)
+    if op == "mul":
+        return srv.mul_i64(key, *args)
+    if op == "dot":
+        return srv.dot_i64(key, *args)
+    return srv.sum(key, *args)
+
+
+def test_alone_equals_mix(eng, ctx, keys):
+    reqs = make_requests(eng, keys, 40, 7)
+    ctx.sync()
+    srv = eng.Server(ctx, workspace_bytes=4 << 30, max_inflight=2)
+    print("server: submitting 40 requests", flush=True)
+    pending = [(op, key, args, *submit(srv, key, op, args)) for op, key, args in reqs]
+    lat = [srv.wait(t, 60000.0) for _, _, _, t, _ in pending]
+    st = srv.stats()
+    print(f"server stats {st}", flush=True)
+    assert st["completed"] == 40 and st["failed"] == 0
+    assert st["max_batch_requests"] > 1  # requests were co-batched
+    assert all(x >= 0 for x in lat)
+    for op, key, args, _, out in pending:
+        want = alone(key, op, args)
+        ctx.sync()
+        assert np.array_equal(host(out), host(want)), op
+    srv.close()
+
+
+def test_server_admission_limits(eng, ctx, keys):
+    # a workspace too small for a big decryption fails that request and keeps serving others
+    key = keys[0]
+    m = np.arange(-50, 50, dtype=np.int64)
+    ct = key.encrypt_i64(m, eng.seeded_r(3, 3, len(m), 256))
+    big = key.encrypt_i64(np.zeros(200000, np.int64), eng.seeded_r(3, 4, 200000, 256))
+    ctx.sync()
+    srv = eng.Server(ctx, workspace_bytes=64 << 20, max_inflight=1, max_batch_units=4)
+    t_big, _ = srv.decrypt(key, big, out_words=2)
+    t_ok, out = srv.decrypt(key, ct, out_words=2)
+    from heu_amd._native import HeuError
+
+    with pytest.raises(HeuError):
+        srv.wait(t_big, 60000.0)
+    srv.wait(t_ok, 60000.0)
+    assert np.array_equal(host(out).view(np.int64).ravel(), m)
+    with pytest.raises(HeuError):
+        srv.wait(12345)
+    srv.close()
