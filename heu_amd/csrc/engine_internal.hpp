@@ -75,6 +75,9 @@ struct PlanOut {
 };
 // Checks the tasks and plans them (caller holds ctx->mu).
 int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, PlanOut* out);
+// Work-list units of `shape` that run at once on the whole device: resident
+// workgroups per CU x CUs x the context's grid multiplier (caller holds ctx->mu).
+long worklist_round_units(heu_ctx* ctx, const heu_hip::WorklistShape* shape, int key_bits, bool dec);
 
 }  // namespace heu_engine
 

@@ -1,11 +1,26 @@
 // server.cpp -- the request server (heu_server_* in include/heu_paillier.h).
 //
-// One scheduler thread per server.  Each turn it retires finished batches
-// (stream events), then, while a stream slot is free and requests are queued,
-// admits the longest prefix of the queue whose batch plan fits the largest
-// free block of the workspace arena (and the unit cap), plans it into that
-// block, copies its work lists up from pinned staging and enqueues it on the
-// slot's stream.  Requests arriving meanwhile queue for the next free slot.
+// One scheduler thread per server.  Work is budgeted in device rounds: a
+// round is the number of work-list units a kernel shape runs at once, and a
+// unit's latency is fixed by its op, so a batch holding less than a round of a
+// shape leaves the chip idle for as long as a full one takes.  Each turn the
+// scheduler retires finished batches (stream events), then, while a stream
+// slot is free, builds the next batch:
+//   - order the active requests by remaining work, shortest first, aged (the
+//     key shrinks the longer a request waits, so long ones are not starved);
+//   - the batch takes the kernel shape of the first request whose shape has
+//     room (each shape may hold batch_rounds rounds in flight across all slots,
+//     decryption only a share of that, which keeps CUs free for the short ops
+//     that arrive during a decryption's long units);
+//   - round-robin over the requests of that shape, each pass carving the next
+//     contiguous range of up to a quantum (1/parts_per_round of a round) from
+//     every request, until the room is used (a sum or dot is taken whole);
+//   - trim to the longest prefix whose plan fits the largest free block of the
+//     workspace arena, plan it into that block, copy its work lists up from
+//     pinned staging and enqueue it on the slot's stream.
+// A long request therefore advances a slice per batch while short requests
+// that arrive meanwhile join the next batch; it completes when its last
+// element does.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,21 +45,80 @@ namespace {
 
 enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 
+// Waiting this long halves a request's remaining-work priority key.
+constexpr double kAgingMs = 100.0;
+// Share of a round that decryption batches may hold at once: a decryption unit
+// runs for tens of milliseconds, so the rest of the chip is kept for the short
+// ops that arrive meanwhile.
+constexpr double kDecRoundShare = 0.875;
+
 struct Request {
   heu_task task;
-  std::vector<int64_t> offsets;  // owned copy for SUM / DOT
+  std::vector<int64_t> offsets;                   // owned copy for SUM / DOT
+  const heu_hip::WorklistShape* shape = nullptr;  // the kernel shape its units run on
+  long round = 1;                                 // units that shape runs at once
+  long gpu = 1;                                   // elements per unit
+  long factor = 1;                                // units per group of elements (2 CRT halves)
+  bool whole = false;                             // SUM / DOT: one part
+  long whole_units = 0;
+  long n = 0, admitted = 0, finished = 0;         // elements (1 for a whole request)
+  size_t sa = 0, sb = 0, so = 0;                  // bytes per element of a, b and out
   Clock::time_point t_submit, t_done;
   int state = kQueued;
   std::string error;
+
+  long units_for(long elems) const { return whole ? whole_units : (elems + gpu - 1) / gpu * factor; }
+  long remaining_units() const { return whole ? (admitted ? 0 : whole_units) : units_for(n - admitted); }
+  heu_task part(long begin, long count) const {
+    if (whole) return task;
+    heu_task p = task;
+    p.count = (size_t)count;
+    p.a = task.a ? static_cast<const char*>(task.a) + begin * sa : nullptr;
+    p.b = task.b ? static_cast<const char*>(task.b) + begin * sb : nullptr;
+    p.out = static_cast<char*>(task.out) + begin * so;
+    return p;
+  }
 };
+
+// Fills in the request's shape and geometry (caller holds ctx->mu).
+void describe(heu_ctx* ctx, Request& r) {
+  const heu_task& t = r.task;
+  const bool dec = t.op == HEU_OP_DECRYPT;
+  r.shape = dec ? &t.key->ops->wl_p : &t.key->ops->wl_n;
+  r.round = std::max<long>(1, worklist_round_units(ctx, r.shape, t.key->key_bits, dec));
+  r.gpu = r.shape->groups_per_unit;
+  r.factor = dec ? 2 : 1;
+  const bool seg = t.op == HEU_OP_SUM || t.op == HEU_OP_DOT_I64;
+  r.whole = seg || t.count == 0;
+  if (r.whole) {
+    const long n = seg && t.segments ? (long)(t.offsets[t.segments] - t.offsets[0]) : 0;
+    const long chunk = t.op == HEU_OP_DOT_I64 ? 8 : 32;  // level-0 chunk of the reduction
+    r.whole_units = (n / chunk + (long)t.segments + r.gpu - 1) / r.gpu;
+    r.n = 1;
+    return;
+  }
+  r.n = (long)t.count;
+  const size_t ct = (size_t)t.key->ct_words * 4;
+  r.sa = r.sb = r.so = ct;
+  switch (t.op) {
+    case HEU_OP_ENCRYPT_I64: r.sa = 8; r.sb = (size_t)t.aux * 4; break;
+    case HEU_OP_RANDOMIZE: r.sb = (size_t)t.aux * 4; break;
+    case HEU_OP_ADD_I64:
+    case HEU_OP_MUL_I64: r.sb = 8; break;
+    case HEU_OP_DECRYPT: r.sb = 0; r.so = (size_t)t.aux * 4; break;
+    default: break;
+  }
+}
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
+  const heu_hip::WorklistShape* shape = nullptr;  // every batch runs one kernel shape
+  long units = 0;
   size_t off = 0, bytes = 0;
-  std::vector<uint64_t> tickets;
-  char* pinned = nullptr;  // staging of the batch's work lists
+  std::vector<std::pair<uint64_t, long>> parts;  // (ticket, elements)
+  char* pinned = nullptr;                         // staging of the batch's work lists
   size_t pinned_cap = 0;
 };
 
@@ -91,6 +165,12 @@ struct Arena {
 
 size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// One part of a batch under construction.
+struct Pick {
+  uint64_t ticket;
+  long begin, count;  // elements (whole requests: 0, 1)
+};
+
 }  // namespace
 
 struct heu_server {
@@ -101,7 +181,7 @@ struct heu_server {
   size_t in_use = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<uint64_t> pending;
+  std::deque<uint64_t> pending;  // requests with elements not yet admitted
   std::unordered_map<uint64_t, Request> reqs;
   uint64_t next_ticket = 1;
   std::vector<Slot> slots;
@@ -112,12 +192,15 @@ struct heu_server {
   void run();
   bool retire();
   bool admit(std::unique_lock<std::mutex>& lk);
+  bool select(std::vector<Pick>* picks, const heu_hip::WorklistShape** shape);
   void fail_request(uint64_t t, const std::string& why) {
     Request& r = reqs[t];
+    if (r.state == kFailed) return;
     r.state = kFailed;
     r.error = why;
     r.t_done = Clock::now();
     ++st.failed;
+    pending.erase(std::remove(pending.begin(), pending.end(), t), pending.end());
   }
 };
 
@@ -128,21 +211,105 @@ bool heu_server::retire() {
     if (!s.busy || hipEventQuery(s.done) != hipSuccess) continue;
     std::lock_guard<std::mutex> lk(mu);
     const auto now = Clock::now();
-    for (uint64_t t : s.tickets) {
-      auto it = reqs.find(t);
-      if (it == reqs.end()) continue;
-      it->second.state = kDone;
-      it->second.t_done = now;
-      ++st.completed;
+    for (const auto& pt : s.parts) {
+      auto it = reqs.find(pt.first);
+      if (it == reqs.end() || it->second.state == kFailed) continue;
+      Request& r = it->second;
+      r.finished += pt.second;
+      if (r.finished == r.n) {
+        r.state = kDone;
+        r.t_done = now;
+        ++st.completed;
+      }
     }
     arena.give(s.off, s.bytes);
     in_use -= s.bytes;
     s.busy = false;
-    s.tickets.clear();
+    s.parts.clear();
     any = true;
   }
   if (any) cv_done.notify_all();
   return any;
+}
+
+// Picks the parts of the next batch (called with mu held).  False if no
+// request's shape has room.
+bool heu_server::select(std::vector<Pick>* picks, const heu_hip::WorklistShape** shape_out) {
+  std::vector<std::pair<double, uint64_t>> order;
+  const auto now = Clock::now();
+  for (uint64_t t : pending) {
+    const Request& r = reqs[t];
+    const double age_ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
+    order.emplace_back((double)r.remaining_units() / (double)r.round / (1.0 + age_ms / kAgingMs), t);
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<double, uint64_t>& x, const std::pair<double, uint64_t>& y) {
+                     return x.first < y.first;
+                   });
+  auto inflight = [&](const heu_hip::WorklistShape* sh) {
+    long u = 0;
+    for (const Slot& s : slots)
+      if (s.busy && s.shape == sh) u += s.units;
+    return u;
+  };
+  auto cap = [&](const Request& r) {
+    const bool dec = r.task.op == HEU_OP_DECRYPT;
+    return std::max<long>(1, (long)((double)r.round * cfg.batch_rounds * (dec ? kDecRoundShare : 1.0)));
+  };
+  const heu_hip::WorklistShape* shape = nullptr;
+  long room = 0, quantum = 1;
+  bool idle = false;
+  for (const auto& ot : order) {
+    const Request& r = reqs[ot.second];
+    const long busy = inflight(r.shape);
+    if (busy < cap(r)) {
+      shape = r.shape;
+      room = cap(r) - busy;
+      idle = busy == 0;
+      quantum = std::max<long>(r.factor, r.round / (long)cfg.parts_per_round);
+      break;
+    }
+  }
+  if (!shape) return false;
+  std::unordered_map<uint64_t, size_t> mine;  // ticket -> index in picks (one merged range each)
+  long used = 0;
+  for (bool added = true; added && picks->size() < cfg.max_batch_tasks;) {
+    added = false;
+    for (const auto& ot : order) {
+      const uint64_t t = ot.second;
+      const Request& r = reqs[t];
+      if (r.shape != shape) continue;
+      auto at = mine.find(t);
+      const bool first = picks->empty() && idle;
+      if (r.whole) {
+        if (r.admitted || at != mine.end()) continue;
+        if (!first && used + r.whole_units > room) continue;
+        mine[t] = picks->size();
+        picks->push_back(Pick{t, 0, 1});
+        used += r.whole_units;
+        added = true;
+      } else {
+        const long have = at == mine.end() ? 0 : (*picks)[at->second].count;
+        const long rem = r.n - r.admitted - have;
+        if (rem <= 0) continue;
+        const long avail = first ? quantum : std::min(quantum, room - used);
+        const long elems = std::min(rem, avail / r.factor * r.gpu);
+        if (elems <= 0) continue;
+        const long before = r.units_for(have);
+        if (at == mine.end()) {
+          mine[t] = picks->size();
+          picks->push_back(Pick{t, r.admitted, elems});
+        } else {
+          (*picks)[at->second].count += elems;
+        }
+        used += r.units_for(have + elems) - before;
+        added = true;
+      }
+      if (picks->size() >= cfg.max_batch_tasks) break;
+    }
+  }
+  *shape_out = shape;
+  return !picks->empty();
 }
 
 // Admits one batch into a free slot if possible (called with `lk` held; may
@@ -156,29 +323,29 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
       break;
     }
   if (!slot || pending.empty()) return false;
-  const size_t n = std::min<size_t>(pending.size(), cfg.max_batch_tasks);
-  std::vector<uint64_t> tickets(pending.begin(), pending.begin() + n);
+  std::vector<Pick> picks;
+  const heu_hip::WorklistShape* shape = nullptr;
+  if (!select(&picks, &shape)) return false;
   std::vector<heu_task> tasks;
-  for (uint64_t t : tickets) tasks.push_back(reqs[t].task);
-  const size_t room = arena.largest(), total = ws.bytes;
+  for (const Pick& p : picks) tasks.push_back(reqs[p.ticket].part(p.begin, p.count));
+  const size_t n = tasks.size();
+  const size_t free_bytes = arena.largest(), total = ws.bytes;
   lk.unlock();
-  // longest admissible prefix: bytes fit the largest free block, units the cap
+  // longest admissible prefix: its plan fits the largest free block
   auto size_of = [&](size_t k, PlanOut* p) {
     std::lock_guard<std::mutex> cl(ctx->mu);
     return plan_batch(ctx, tasks.data(), k, nullptr, p);
   };
-  auto fits = [&](const PlanOut& p, size_t k) {
-    return round256(p.bytes) <= room && (cfg.max_batch_units == 0 || p.units <= cfg.max_batch_units || k == 1);
-  };
+  auto fits = [&](const PlanOut& p) { return round256(p.bytes) <= free_bytes; };
   size_t k = n;
   PlanOut p;
   int rc = size_of(k, &p);
-  if (rc == HEU_OK && !fits(p, k)) {
-    size_t lo = 0, hi = n;  // fits(lo) known true (or lo == 0), fits(hi) false
+  if (rc == HEU_OK && !fits(p)) {
+    size_t lo = 0, hi = n;  // fits(lo) holds (or lo == 0), fits(hi) does not
     while (hi - lo > 1) {
       const size_t mid = (lo + hi) / 2;
       PlanOut q;
-      if (size_of(mid, &q) == HEU_OK && fits(q, mid)) lo = mid;
+      if (size_of(mid, &q) == HEU_OK && fits(q)) lo = mid;
       else hi = mid;
     }
     k = lo;
@@ -186,14 +353,14 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   }
   lk.lock();
   if (rc != HEU_OK || k == 0) {
-    // the head request alone: fail it if it can never fit, else wait for room
+    // the first part alone: fail its request if it can never fit, else wait for room
     PlanOut one;
     lk.unlock();
     const int rc1 = size_of(1, &one);
     lk.lock();
     if (rc1 != HEU_OK || round256(one.bytes) > total) {
-      fail_request(tickets[0], rc1 != HEU_OK ? heu_last_error() : "request needs more workspace than the server has");
-      pending.pop_front();
+      fail_request(picks[0].ticket,
+                   rc1 != HEU_OK ? heu_last_error() : "request needs more workspace than the server has");
       cv_done.notify_all();
       return true;
     }
@@ -204,14 +371,21 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   if (!arena.take(bytes, &off)) return false;
   in_use += bytes;
   st.peak_workspace_bytes = std::max(st.peak_workspace_bytes, in_use);
+  long units = 0;
+  slot->parts.clear();
   for (size_t i = 0; i < k; ++i) {
-    pending.pop_front();
-    reqs[tickets[i]].state = kRunning;
+    Request& r = reqs[picks[i].ticket];
+    r.state = kRunning;
+    units += r.units_for(picks[i].count);
+    r.admitted += picks[i].count;
+    slot->parts.emplace_back(picks[i].ticket, picks[i].count);
+    if (r.admitted == r.n) pending.erase(std::find(pending.begin(), pending.end(), picks[i].ticket));
   }
   slot->busy = true;
+  slot->shape = shape;
+  slot->units = units;
   slot->off = off;
   slot->bytes = bytes;
-  slot->tickets.assign(tickets.begin(), tickets.begin() + k);
   ++st.batches;
   st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, k);
   lk.unlock();
@@ -248,8 +422,8 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   if (rc != HEU_OK) {
     // nothing (or not all) was enqueued: the requests fail; the slot retires
     // through its event as usual
-    for (uint64_t t : slot->tickets) fail_request(t, "could not enqueue the batch");
-    slot->tickets.clear();
+    for (const auto& pt : slot->parts) fail_request(pt.first, "could not enqueue the batch");
+    slot->parts.clear();
     cv_done.notify_all();
   }
   return true;
@@ -280,8 +454,10 @@ int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** o
   srv->ctx = ctx;
   if (cfg) srv->cfg = *cfg;
   if (!srv->cfg.workspace_bytes) srv->cfg.workspace_bytes = (size_t)8 << 30;
-  if (!srv->cfg.max_inflight) srv->cfg.max_inflight = 2;
+  if (!srv->cfg.max_inflight) srv->cfg.max_inflight = 4;
   if (!srv->cfg.max_batch_tasks) srv->cfg.max_batch_tasks = 512;
+  if (!srv->cfg.batch_rounds) srv->cfg.batch_rounds = 1;
+  if (!srv->cfg.parts_per_round) srv->cfg.parts_per_round = 8;
   if (srv->cfg.max_inflight > 8) return fail(HEU_EINVAL, "max_inflight must be at most 8");
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = srv->ws.alloc(round256(srv->cfg.workspace_bytes));
@@ -313,14 +489,15 @@ int heu_server_submit(heu_server* srv, const heu_task* task, uint64_t* ticket) {
     std::lock_guard<std::mutex> cl(srv->ctx->mu);
     int rc = plan_batch(srv->ctx, &r.task, 1, nullptr, &chk);
     if (rc) return rc;
+    describe(srv->ctx, r);
   }
   std::lock_guard<std::mutex> lk(srv->mu);
   if (srv->stop) return fail(HEU_EINVAL, "server is stopping");
   const uint64_t t = srv->next_ticket++;
   r.t_submit = Clock::now();
-  auto& slot = srv->reqs[t];
-  slot = std::move(r);
-  if (!slot.offsets.empty()) slot.task.offsets = slot.offsets.data();  // the vector moved
+  Request& stored = srv->reqs[t];
+  stored = std::move(r);
+  if (!stored.offsets.empty()) stored.task.offsets = stored.offsets.data();  // the vector moved
   srv->pending.push_back(t);
   ++srv->st.submitted;
   *ticket = t;

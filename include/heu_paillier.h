@@ -201,7 +201,13 @@ int heu_ctx_set_grid_multiplier(heu_ctx *ctx, uint32_t key_bits, int decrypt_sha
  * runs up to max_inflight batches at once on streams of its own, and retires
  * them as they finish: requests that arrive while others run are batched and
  * started as soon as a stream slot and workspace are free, so their work
- * overlaps on the device.  A request's result is independent of what it was
+ * overlaps on the device.  Elementwise requests are cut into parts of a
+ * fraction of a round of the device (the work units its persistent launch runs
+ * at once), and every batch takes parts round-robin across the active requests,
+ * shortest remaining work first, so
+ * a long request shares the device with the short ones that arrive during it
+ * instead of blocking them (sums and dots stay whole).  A request completes
+ * when its last part does, and its result is independent of what it was
  * batched with.  HEU has no counterpart; it serves the request-loop shape of
  * the path (keys and tables resident, requests of mixed sizes, keys and ops). */
 typedef struct heu_server heu_server;
@@ -209,8 +215,10 @@ typedef struct heu_server heu_server;
 typedef struct heu_server_config {
   size_t workspace_bytes;   /* device workspace for in-flight batches (0: 8 GiB) */
   uint32_t max_inflight;    /* batches on the device at once (0: 2) */
-  uint32_t max_batch_tasks; /* requests per batch (0: 512) */
-  uint64_t max_batch_units; /* work-list units per batch, bounds batch latency (0: no bound) */
+  uint32_t max_batch_tasks; /* request parts per batch (0: 512) */
+  uint32_t batch_rounds;    /* work-list units per batch and kernel shape, in rounds of the
+                               units the device runs at once (0: 1) */
+  uint32_t parts_per_round; /* elementwise requests are cut into parts of 1/parts_per_round of a round (0: 4) */
 } heu_server_config;
 
 typedef struct heu_server_counters {

@@ -112,13 +112,14 @@ def test_alone_equals_mix(eng, ctx, keys):
 
 
 def test_server_admission_limits(eng, ctx, keys):
-    # a workspace too small for a big decryption fails that request and keeps serving others
+    # a workspace too small for even one part of a big decryption fails that
+    # request and keeps serving others
     key = keys[0]
     m = np.arange(-50, 50, dtype=np.int64)
     ct = key.encrypt_i64(m, eng.seeded_r(3, 3, len(m), 256))
     big = key.encrypt_i64(np.zeros(200000, np.int64), eng.seeded_r(3, 4, 200000, 256))
     ctx.sync()
-    srv = eng.Server(ctx, workspace_bytes=64 << 20, max_inflight=1, max_batch_units=4)
+    srv = eng.Server(ctx, workspace_bytes=8 << 20, max_inflight=1)
     t_big, _ = srv.decrypt(key, big, out_words=2)
     t_ok, out = srv.decrypt(key, ct, out_words=2)
     from heu_amd._native import HeuError
@@ -129,4 +130,26 @@ def test_server_admission_limits(eng, ctx, keys):
     assert np.array_equal(host(out).view(np.int64).ravel(), m)
     with pytest.raises(HeuError):
         srv.wait(12345)
+    srv.close()
+
+
+def test_long_request_does_not_block_short_ones(eng, ctx, keys):
+    # a long decryption is cut into parts; short requests submitted after it
+    # finish long before it does, and every result is exact
+    key = keys[1]  # 3072-bit
+    n_big = 60000
+    big_m = np.arange(n_big, dtype=np.int64) - n_big // 2
+    big = key.encrypt_i64(big_m, eng.seeded_r(9, 1, n_big, 256))
+    small = [key.encrypt_i64(np.array([i, -i, 7 * i], np.int64), eng.seeded_r(9, 10 + i, 3, 256)) for i in range(8)]
+    ctx.sync()
+    srv = eng.Server(ctx, max_inflight=2)
+    t_big, out_big = srv.decrypt(key, big, out_words=2)
+    subs = [srv.decrypt(key, c, out_words=2) for c in small]
+    lat_small = [srv.wait(t, 120000.0) for t, _ in subs]
+    lat_big = srv.wait(t_big, 120000.0)
+    print(f"big {lat_big:.1f} ms, small max {max(lat_small):.1f} ms", flush=True)
+    assert max(lat_small) < lat_big
+    assert np.array_equal(host(out_big).view(np.int64).ravel(), big_m)
+    for i, (_, out) in enumerate(subs):
+        assert np.array_equal(host(out).view(np.int64).ravel(), np.array([i, -i, 7 * i], np.int64))
     srv.close()
