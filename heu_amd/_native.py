@@ -49,6 +49,9 @@ SIGNATURES = {
     "heu_batch_destroy": (I, [P]),
     "heu_batch_stats": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(SZ)]),
     "heu_ctx_set_grid_multiplier": (I, [P, U32, I, U32]),
+    "heu_key_set_param": (I, [P, I, U32]),
+    "heu_key_get_param": (I, [P, I, ctypes.POINTER(U32)]),
+    "heu_autotune": (I, [P, U32, ctypes.c_char_p, SZ]),
     "heu_server_create": (I, [P, P, ctypes.POINTER(P)]),
     "heu_server_submit": (I, [P, P, ctypes.POINTER(U64)]),
     "heu_server_wait": (I, [P, U64, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
@@ -60,6 +63,8 @@ SIGNATURES = {
 (HEU_OP_ENCRYPT_I64, HEU_OP_RANDOMIZE, HEU_OP_ADD, HEU_OP_ADD_I64, HEU_OP_MUL_I64, HEU_OP_DECRYPT, HEU_OP_SUM,
  HEU_OP_DOT_I64, HEU_OP_BARRIER) = range(1, 10)
 HEU_BATCH_GRAPH = 1
+HEU_PARAM_TABLE_WINDOW, HEU_PARAM_DEC_LANES = 1, 2
+HEU_TUNE_TABLE, HEU_TUNE_DEC_LANES, HEU_TUNE_GRID, HEU_TUNE_ALL = 1, 2, 4, 7
 
 
 class HeuTask(ctypes.Structure):

@@ -1,10 +1,10 @@
 """Builds the engine library heu_amd/libheu_paillier.so for gfx950 with hipcc.
 
 Sources: csrc/ops_{2048,3072}.hip (kernels and launch tables per key size) and
-csrc/worklist_{2048,3072}.hip (persistent work-list kernels), all built from
+csrc/worklist_{,dec_}{2048,3072}.hip (persistent work-list kernels), all built from
 csrc/paillier_device.hpp; csrc/engine.cpp (contexts, keys, one-op entry
 points), csrc/batch.cpp (multi-task batches, graphs), csrc/server.cpp (request
-server) and csrc/keygen.cpp.
+server), csrc/tune.cpp (autotuner) and csrc/keygen.cpp.
 The library links OpenSSL's libcrypto for per-key host constants.
 """
 from __future__ import annotations
@@ -19,8 +19,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libheu_paillier.so")
 ARCH = os.environ.get("HEU_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["ops_2048.hip", "ops_3072.hip", "worklist_2048.hip", "worklist_3072.hip", "engine.cpp",
-           "batch.cpp", "server.cpp", "keygen.cpp"]  # valu_peak.hip builds separately
+SOURCES = ["ops_2048.hip", "ops_3072.hip", "worklist_2048.hip", "worklist_3072.hip", "worklist_dec_2048.hip",
+           "worklist_dec_3072.hip", "engine.cpp", "batch.cpp", "server.cpp", "tune.cpp",
+           "keygen.cpp"]  # valu_peak.hip builds separately
 # Packed FP32 instructions stay off in device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -260,7 +260,7 @@ int resident_per_cu(heu_ctx* ctx, const WorklistShape* s) {
 int plan_stage(Builder& b, heu_ctx* ctx, const heu_task* tasks, size_t ntasks, uint64_t* units_out) {
   std::map<ShapeKey, WorkList> lists;
   auto list_for = [&](heu_key* key, bool dec) -> WorkList& {
-    const WorklistShape* s = dec ? &key->ops->wl_p : &key->ops->wl_n;
+    const WorklistShape* s = dec ? &key->dshape().wl : &key->ops->wl_n;
     WorkList& wl = lists[ShapeKey{s}];
     wl.shape = s;
     wl.key_bits = key->key_bits;

@@ -199,6 +199,43 @@ int upload_mod(const BIGNUM* m, int L, int K, DevBuf& dm, DevBuf& done, DevBuf& 
   return HEU_OK;
 }
 
+// Builds (or rebuilds) the fixed-base table of h_s for a window of
+// `window_bits` (public_key.cc:36-39 MakeBaseTable): T[w][d] = h_s^(d 2^(unit w)) R
+// for every window w covering key->r_bits, built on the device by levels.
+int build_table(heu_key* key, uint32_t window_bits) {
+  heu_ctx* ctx = key->ctx;
+  const Ops* ops = key->ops;
+  hipStream_t st = ctx->stream;
+  BnCtx bc;
+  Bn b, n2;
+  bn_from_words(b, key->hs_host.data(), key->hs_host.size());
+  bn_from_words(n2, key->n2_host.data(), key->n2_host.size());
+  const int windows = (int)((key->r_bits + window_bits - 1) / window_bits);
+  std::vector<uint32_t> bases((size_t)windows * key->ct_words);
+  for (int w = 0; w < windows; ++w) {
+    auto v = bn_to_words(b, key->ct_words);
+    std::copy(v.begin(), v.end(), bases.begin() + (size_t)w * key->ct_words);
+    for (uint32_t s = 0; s < window_bits; ++s) BN_mod_sqr(b, b, n2, bc.c);
+  }
+  DevBuf dbases;
+  size_t dummy = 0;
+  int rc = upload(dbases, bases, st, &dummy);
+  if (rc) return rc;
+  HIP_TRY(hipDeviceSynchronize());  // no launch may still read the old table
+  key->device_bytes -= key->table.bytes;
+  const size_t tbytes = ((size_t)windows << window_bits) * key->nl_n * 4;
+  if ((rc = key->table.alloc(tbytes))) return rc;
+  key->device_bytes += tbytes;
+  key->unit = (int)window_bits;
+  key->windows = windows;
+  for (int level = 0; level <= (int)window_bits; ++level)
+    LAUNCH(ctx, "table_level", ops->table_level(st, key->mod_n2(), dbases.as<uint32_t>(), key->table.as<uint32_t>(),
+                                                windows, key->unit, level));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return HEU_OK;
+}
+
 // Carves 256-aligned parts from the context's scratch pool, starting at
 // `base`.  The pool is reallocated (after draining the stream) only by a
 // top-level carve (base == 0); nested carves must fit what the caller reserved.
@@ -499,7 +536,7 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
   key->ct_words = ops->ct_words;
   key->prime_words = ops->p_words;
   key->nl_n = ops->n_L * ops->n_K;
-  key->nl_p = ops->p_L * ops->p_K;
+  key->r_bits = r_bits;
   hipStream_t st = ctx->stream;
   size_t* tb = &key->device_bytes;
   int rc = upload_mod(n2, ops->n_L, ops->n_K, key->n2_m, key->n2_one, key->n2_r2, nullptr, &key->n2_mprime, st, tb, bc.c);
@@ -527,29 +564,8 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
       }
     }
   }
-  key->unit = (int)window_bits;
-  key->windows = (int)((r_bits + window_bits - 1) / window_bits);
-  {
-    std::vector<uint32_t> bases((size_t)key->windows * key->ct_words);
-    Bn b;
-    BN_copy(b, hs);
-    for (int w = 0; w < key->windows; ++w) {
-      auto v = bn_to_words(b, key->ct_words);
-      std::copy(v.begin(), v.end(), bases.begin() + (size_t)w * key->ct_words);
-      for (uint32_t s = 0; s < window_bits; ++s) BN_mod_sqr(b, b, n2, bc.c);
-    }
-    DevBuf dbases;
-    size_t dummy = 0;
-    if ((rc = upload(dbases, bases, st, &dummy))) return rc;
-    const size_t tbytes = ((size_t)key->windows << window_bits) * key->nl_n * 4;
-    if ((rc = key->table.alloc(tbytes))) return rc;
-    key->device_bytes += tbytes;
-    for (int level = 0; level <= (int)window_bits; ++level)
-      LAUNCH(ctx, "table_level", ops->table_level(st, key->mod_n2(), dbases.as<uint32_t>(), key->table.as<uint32_t>(), key->windows,
-                       key->unit, level));
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
-  }
+  key->hs_host = bn_to_words(hs, key->ct_words);
+  if ((rc = build_table(key.get(), window_bits))) return rc;
   if (p_w && q_w) {
     Bn p, q, t, p2, q2;
     bn_from_words(p, p_w, prime_words);
@@ -560,12 +576,12 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
       return fail(HEU_EINVAL, "prime wider than %d words", ops->p_words);
     BN_sqr(p2, p, bc.c);
     BN_sqr(q2, q, bc.c);
-    if ((rc = upload_mod(p2, ops->p_L, ops->p_K, key->p2_m, key->p2_one, key->p2_r2, &key->p2_r3, &key->p2_mprime, st,
-                         tb, bc.c)))
-      return rc;
-    if ((rc = upload_mod(q2, ops->p_L, ops->p_K, key->q2_m, key->q2_one, key->q2_r2, &key->q2_r3, &key->q2_mprime, st,
-                         tb, bc.c)))
-      return rc;
+    for (int v = 0; v < heu_hip::kDecShapes; ++v) {  // constants of every decryption shape
+      const heu_hip::DecShape& d = ops->dec[v];
+      heu_key::Half &hp = key->p2[v], &hq = key->q2[v];
+      if ((rc = upload_mod(p2, d.L, d.K, hp.m, hp.one, hp.r2, &hp.r3, &hp.mprime, st, tb, bc.c))) return rc;
+      if ((rc = upload_mod(q2, d.L, d.K, hq.m, hq.one, hq.r2, &hq.r3, &hq.mprime, st, tb, bc.c))) return rc;
+    }
     Bn pm1, qm1;
     BN_copy(pm1, p);
     BN_sub_word(pm1, 1);
@@ -631,6 +647,38 @@ int heu_key_info(const heu_key* key, uint32_t* key_bits, uint32_t* n_words, uint
 }
 
 size_t heu_key_device_bytes(const heu_key* key) { return key ? key->device_bytes : 0; }
+
+int heu_key_set_param(heu_key* key, int param, uint32_t value) {
+  int rc = check_key(key);
+  if (rc) return rc;
+  heu_ctx* ctx = key->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  switch (param) {
+    case HEU_PARAM_TABLE_WINDOW:
+      if (value < 1 || value > 16) return fail(HEU_EINVAL, "table window must be 1..16 bits");
+      return build_table(key, value);
+    case HEU_PARAM_DEC_LANES:
+      for (int v = 0; v < heu_hip::kDecShapes; ++v)
+        if ((uint32_t)key->ops->dec[v].L == value) {
+          HIP_TRY(hipDeviceSynchronize());
+          key->dec_shape = v;
+          return HEU_OK;
+        }
+      return fail(HEU_EINVAL, "no decryption shape with %u lanes for %d-bit keys", value, key->key_bits);
+    default:
+      return fail(HEU_EINVAL, "unknown parameter %d", param);
+  }
+}
+
+int heu_key_get_param(const heu_key* key, int param, uint32_t* value) {
+  if (!key || !value) return fail(HEU_EINVAL, "null argument");
+  switch (param) {
+    case HEU_PARAM_TABLE_WINDOW: *value = (uint32_t)key->unit; return HEU_OK;
+    case HEU_PARAM_DEC_LANES: *value = (uint32_t)key->dshape().L; return HEU_OK;
+    default: return fail(HEU_EINVAL, "unknown parameter %d", param);
+  }
+}
 
 #define KEY_PROLOGUE(key)                          \
   int rc_ = check_key(key);                        \
@@ -703,14 +751,14 @@ int heu_decrypt(heu_key* key, const uint32_t* ct, size_t count, uint32_t* out, u
   const size_t HWB = (size_t)key->ct_words / 2 * 4;
   Scratch sc(ctx);
   size_t halves = sc.add(2 * (size_t)slice * HWB);
-  size_t tab = sc.add(2 * (size_t)slice * ((size_t)key->nl_p * 4 << (heu_hip::kDecWindowBits - 1)));
+  size_t tab = sc.add(2 * (size_t)slice * ((size_t)key->nl_p() * 4 << (heu_hip::kDecWindowBits - 1)));
   int rc = sc.commit();
   if (rc) return rc;
   const DecArgs D = key->dec();
   const CombArgs C = key->comb();
   for (long s0 = 0; s0 < (long)count; s0 += slice) {
     const long n = std::min<long>(slice, (long)count - s0);
-    LAUNCH(ctx, "dec_half", o->dec_half(ctx->stream, D, ct + s0 * key->ct_words, n, sc.at<uint32_t>(halves), sc.at<uint32_t>(tab)));
+    LAUNCH(ctx, "dec_half", key->dshape().dec_half(ctx->stream, D, ct + s0 * key->ct_words, n, sc.at<uint32_t>(halves), sc.at<uint32_t>(tab)));
     LAUNCH(ctx, "dec_combine", o->combine(ctx->stream, C, sc.at<uint32_t>(halves), n, out + s0 * out_words, (int)out_words,
                overflow ? overflow + s0 : nullptr));
   }

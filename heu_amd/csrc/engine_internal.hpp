@@ -149,31 +149,39 @@ struct heu_key {
   heu_ctx* ctx = nullptr;
   const Ops* ops = nullptr;
   int key_bits = 0, n_words = 0, ct_words = 0, prime_words = 0, has_sk = 0;
-  int nl_n = 0, nl_p = 0;
+  int nl_n = 0;
+  uint32_t r_bits = 0;  // randomness bits the table covers
   // public side
   heu_engine::DevBuf n2_m, n2_one, n2_r2, n29, table, n2_words;
   uint32_t n2_mprime = 0;
   int windows = 0, unit = 0;
-  // secret side
-  heu_engine::DevBuf p2_m, p2_one, p2_r2, p2_r3, q2_m, q2_one, q2_r2, q2_r3, sched_p, sched_q;
-  uint32_t p2_mprime = 0, q2_mprime = 0;
+  // secret side: p^2 / q^2 constants for every decryption shape, one active
+  struct Half {
+    heu_engine::DevBuf m, one, r2, r3;
+    uint32_t mprime = 0;
+    ModArgs args() const {
+      return ModArgs{m.as<uint32_t>(), one.as<uint32_t>(), r2.as<uint32_t>(), r3.as<uint32_t>(), mprime};
+    }
+  };
+  Half p2[heu_hip::kDecShapes], q2[heu_hip::kDecShapes];
+  int dec_shape = 0;
+  heu_engine::DevBuf sched_p, sched_q;
   int nsched_p = 0, nsched_q = 0;
   heu_engine::DevBuf c_p, c_q, c_pinv, c_qinv, c_hpR, c_hqR, c_pinvqR, c_n, c_nhalf;
   uint32_t c_pprime = 0, c_qprime = 0;
   // host copies for argument checks and conversions
-  std::vector<uint32_t> n_host, n2_host, nhalf_host;
+  std::vector<uint32_t> n_host, n2_host, nhalf_host, hs_host;
   size_t device_bytes = 0;
 
   ModArgs mod_n2() const {
     return ModArgs{n2_m.as<uint32_t>(), n2_one.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_mprime};
   }
   PubArgs pub() const { return PubArgs{mod_n2(), n29.as<uint32_t>(), table.as<uint32_t>(), windows, unit}; }
+  const heu_hip::DecShape& dshape() const { return ops->dec[dec_shape]; }
+  int nl_p() const { return dshape().L * dshape().K; }
   DecArgs dec() const {
-    return DecArgs{ModArgs{p2_m.as<uint32_t>(), p2_one.as<uint32_t>(), p2_r2.as<uint32_t>(), p2_r3.as<uint32_t>(),
-                           p2_mprime},
-                   ModArgs{q2_m.as<uint32_t>(), q2_one.as<uint32_t>(), q2_r2.as<uint32_t>(), q2_r3.as<uint32_t>(),
-                           q2_mprime},
-                   sched_p.as<uint32_t>(), sched_q.as<uint32_t>(), nsched_p, nsched_q};
+    return DecArgs{p2[dec_shape].args(), q2[dec_shape].args(), sched_p.as<uint32_t>(), sched_q.as<uint32_t>(),
+                   nsched_p, nsched_q};
   }
   CombArgs comb() const {
     return CombArgs{c_p.as<uint32_t>(),   c_q.as<uint32_t>(),    c_pinv.as<uint32_t>(), c_qinv.as<uint32_t>(),

@@ -122,10 +122,21 @@ struct WorklistShape {
   long slot_table_words;    // window-table words per group slot
 };
 
+// One decryption shape: the lane group mod p^2 / q^2 (L lanes x K limbs) and
+// its kernels.  Each key size has kDecShapes of them (lanes per bignum is a
+// tuning choice: fewer lanes, more throughput per lane; more lanes, shorter
+// latency per element).
+struct DecShape {
+  int L, K;
+  DecHalfFn dec_half;
+  WorklistShape wl;
+};
+constexpr int kDecShapes = 2;
+
 // Launchers and shape constants for one key size.
 struct Ops {
   int n_L, n_K, ct_words;  // group shape mod n^2 and ciphertext words
-  int p_L, p_K, p_words;   // group shape mod p^2 and words of p
+  int p_words;             // words of p
   TableLevelFn table_level;
   EncI64Fn encrypt_i64;
   EncBigFn encrypt_big;
@@ -143,10 +154,9 @@ struct Ops {
   InvDownFn inv_down;
   InvSingleFn inv_single;
   MulConstFn mul_const;
-  DecHalfFn dec_half;
   CombineFn combine;
-  WorklistShape wl_n;  // n^2 ops: encrypt, randomize, add, add_pt, mul, dot0, sum0
-  WorklistShape wl_p;  // CRT decryption halves mod p^2 / q^2
+  WorklistShape wl_n;             // n^2 ops: encrypt, randomize, add, add_pt, mul, dot0, sum0
+  DecShape dec[kDecShapes];       // CRT decryption halves mod p^2 / q^2; [0] is the default
 };
 
 const Ops* ops_for_key_bits(int key_bits);

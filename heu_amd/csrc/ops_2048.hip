@@ -1,15 +1,16 @@
-// ops_2048.hip -- launch table for 2048-bit keys (n^2 over 4 lanes x 36 limbs,
-// p^2 over 2 lanes x 36 limbs) and the key-size dispatch.
+// ops_2048.hip -- launch table for 2048-bit keys (n^2 over 4 lanes x 36 limbs;
+// p^2 over 2 lanes x 36 or 4 lanes x 18 limbs) and the key-size dispatch.
 #include "paillier_device.hpp"
 
 namespace heu_hip {
 
 WorklistShape worklist_n2048();
-WorklistShape worklist_p2048();
+WorklistShape worklist_p2048(int shape);
 const Ops* ops_3072();
 
 const Ops* ops_2048() {
-  static const Ops o = make_ops<ShapeN2048, ShapeP2048, 32>(worklist_n2048(), worklist_p2048());
+  static const Ops o = make_ops<ShapeN2048, 32>(worklist_n2048(), Launch<ShapeP2048>::dec_shape(worklist_p2048(0)),
+                                                Launch<ShapeP2048w>::dec_shape(worklist_p2048(1)));
   return &o;
 }
 

@@ -1384,6 +1384,14 @@ struct Launch {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_worklist<L, K, W, kDec>, kBlock, 0) != hipSuccess) return 1;
     return n > 0 ? n : 1;
   }
+  static DecShape dec_shape(const WorklistShape& wl) {
+    DecShape d;
+    d.L = L;
+    d.K = K;
+    d.dec_half = dec_half;
+    d.wl = wl;
+    return d;
+  }
   template <bool kDec>
   static WorklistShape worklist_shape() {
     WorklistShape w;
@@ -1403,24 +1411,26 @@ static void launch_combine(hipStream_t st, const CombArgs& C, const u32* halves,
 }
 
 // Shapes (29-bit limbs, 29*NL >= bits(m) + 2):
-//   2048-bit keys: n^2 over 4 lanes x 36 limbs, p^2 over 2 lanes x 36 limbs;
-//   3072-bit keys: n^2 over 8 lanes x 27 limbs, p^2 over 4 lanes x 27 limbs.
+//   2048-bit keys: n^2 over 4 lanes x 36 limbs, p^2 over 2 lanes x 36 limbs
+//                  (or 4 lanes x 18);
+//   3072-bit keys: n^2 over 8 lanes x 27 limbs, p^2 over 4 lanes x 27 limbs
+//                  (or 8 lanes x 14: NL = 112 >= 106).
 // W = words of the ciphertext (decryption halves hold W/2).
 typedef Shape<4, 36, 128> ShapeN2048;
 typedef Shape<8, 27, 192> ShapeN3072;
 typedef Shape<2, 36, 128> ShapeP2048;
+typedef Shape<4, 18, 128> ShapeP2048w;
 typedef Shape<4, 27, 192> ShapeP3072;
+typedef Shape<8, 14, 192> ShapeP3072w;
 
 // The launch table of one key size; the work-list kernels come from their own
 // translation units (they are the largest kernels to compile).
-template <class SN, class SP, int PW>
-static Ops make_ops(const WorklistShape& wl_n, const WorklistShape& wl_p) {
+template <class SN, int PW>
+static Ops make_ops(const WorklistShape& wl_n, const DecShape& dec0, const DecShape& dec1) {
   Ops o;
   o.n_L = SN::kL;
   o.n_K = SN::kK;
   o.ct_words = SN::kW;
-  o.p_L = SP::kL;
-  o.p_K = SP::kK;
   o.p_words = PW;
   o.table_level = Launch<SN>::table_level;
   o.encrypt_i64 = Launch<SN>::encrypt_i64;
@@ -1439,10 +1449,10 @@ static Ops make_ops(const WorklistShape& wl_n, const WorklistShape& wl_p) {
   o.inv_down = Launch<SN>::inv_down;
   o.inv_single = Launch<SN>::inv_single;
   o.mul_const = Launch<SN>::mul_const;
-  o.dec_half = Launch<SP>::dec_half;
   o.combine = launch_combine<PW>;
   o.wl_n = wl_n;
-  o.wl_p = wl_p;
+  o.dec[0] = dec0;
+  o.dec[1] = dec1;
   return o;
 }
 

@@ -84,7 +84,7 @@ struct Request {
 void describe(heu_ctx* ctx, Request& r) {
   const heu_task& t = r.task;
   const bool dec = t.op == HEU_OP_DECRYPT;
-  r.shape = dec ? &t.key->ops->wl_p : &t.key->ops->wl_n;
+  r.shape = dec ? &t.key->dshape().wl : &t.key->ops->wl_n;
   r.round = std::max<long>(1, worklist_round_units(ctx, r.shape, t.key->key_bits, dec));
   r.gpu = r.shape->groups_per_unit;
   r.factor = dec ? 2 : 1;

@@ -227,6 +227,36 @@ class Key:
             raise ValueError(f"ciphertexts must have shape (N, {self.ct_words})")
         return t.shape[0]
 
+    # ------------------------------------------------------------- tuning
+    @property
+    def table_window(self) -> int:
+        v = ctypes.c_uint32()
+        check(lib().heu_key_get_param(self._h, _native.HEU_PARAM_TABLE_WINDOW, ctypes.byref(v)))
+        return v.value
+
+    @table_window.setter
+    def table_window(self, bits: int) -> None:
+        check(lib().heu_key_set_param(self._h, _native.HEU_PARAM_TABLE_WINDOW, bits))
+
+    @property
+    def dec_lanes(self) -> int:
+        v = ctypes.c_uint32()
+        check(lib().heu_key_get_param(self._h, _native.HEU_PARAM_DEC_LANES, ctypes.byref(v)))
+        return v.value
+
+    @dec_lanes.setter
+    def dec_lanes(self, lanes: int) -> None:
+        check(lib().heu_key_set_param(self._h, _native.HEU_PARAM_DEC_LANES, lanes))
+
+    def autotune(self, what: int = _native.HEU_TUNE_ALL) -> dict:
+        """Times the table window, decryption lanes per bignum and work-list
+        grid multiplier on this device, keeps the fastest; returns the report."""
+        import json
+
+        buf = ctypes.create_string_buffer(8192)
+        check(lib().heu_autotune(self._h, what, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     # ------------------------------------------------------------- ops
     def encrypt_i64(self, m, r) -> torch.Tensor:
         m = self._dev(m, torch.int64)

@@ -88,6 +88,26 @@ int heu_key_info(const heu_key *key, uint32_t *key_bits, uint32_t *n_words, uint
 /* Device bytes held by the key (constants + table). */
 size_t heu_key_device_bytes(const heu_key *key);
 
+/* Tunable per-key parameters (no counterpart in the reference, whose window
+ * is the constant kExpUnitBits = 10, public_key.cc:21):
+ *   HEU_PARAM_TABLE_WINDOW  fixed-base table window in bits (1..16); setting it
+ *                           rebuilds the table (waits for the whole device);
+ *   HEU_PARAM_DEC_LANES     lanes per bignum of the CRT decryption: 2 or 4 for
+ *                           2048-bit keys, 4 or 8 for 3072-bit keys.
+ * No work using the key may be in flight while a parameter is set. */
+enum { HEU_PARAM_TABLE_WINDOW = 1, HEU_PARAM_DEC_LANES = 2 };
+int heu_key_set_param(heu_key *key, int param, uint32_t value);
+int heu_key_get_param(const heu_key *key, int param, uint32_t *value);
+
+/* Autotuner: times the candidates of each parameter on this device with this
+ * key (synthetic inputs) and keeps the fastest -- the table window (encryption
+ * throughput), the decryption lanes per bignum (decryption throughput) and
+ * the persistent work-list grid multiplier (occupancy) of the key's shapes,
+ * which is set on the key's context.  `what` is a mask of HEU_TUNE_*; report
+ * (may be NULL) receives a JSON object with every timing and the choices. */
+enum { HEU_TUNE_TABLE = 1, HEU_TUNE_DEC_LANES = 2, HEU_TUNE_GRID = 4, HEU_TUNE_ALL = 7 };
+int heu_autotune(heu_key *key, uint32_t what, char *report, size_t report_len);
+
 /* ---- batch operations: device pointers, asynchronous ------------------- */
 /* Encryptor::Encrypt (paillier_zahlen/encryptor.cc:35-49) with injected r:
  * ct[i] = (1 + n*m[i]) * h_s^r[i] mod n^2.  r: count x r_words limbs. */
