@@ -42,6 +42,9 @@ SIGNATURES = {
     "heu_mul": (I, [P, P, P, P, U32, SZ, P]),
     "heu_sum": (I, [P, P, P, SZ, P]),
     "heu_dot_i64": (I, [P, P, P, P, SZ, P]),
+    "heu_sum_gather": (I, [P, P, P, P, SZ, P]),
+    "heu_dot_gather": (I, [P, P, P, P, P, SZ, P]),
+    "heu_prefix_mul": (I, [P, P, P, SZ, P]),
     "heu_to_mont": (I, [P, P, SZ, U32, P]),
     "heu_from_mont": (I, [P, P, SZ, U32, P]),
     "heu_batch_create": (I, [P, P, SZ, U32, ctypes.POINTER(P)]),

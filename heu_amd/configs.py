@@ -77,6 +77,7 @@ def _elapsed_ms(fn, ctx) -> float:
 def run_config3(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
     kd = E.keygen(3072, (3 << 16) | seed)
     key = E.Key.from_keygen(ctx, kd, r_bits=R_BITS, window_bits=0)
+    tune = key.autotune()
     lens = log_uniform_lengths(3, seed, 1, 64, 1, 65536)
     total = int(lens.sum())
     offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
@@ -133,6 +134,7 @@ def run_config3(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
         "work_units": st["units"],
         "bit_exact_vs_plain": bool(same),
         "closed_form_check": bool(closed),
+        "autotune": tune,
     }
 
 
@@ -144,6 +146,16 @@ def run_config4(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
     for i in range(16):
         bits = 2048 if i < 8 else 3072
         keys.append(E.Key.from_keygen(ctx, E.keygen(bits, (4 << 16) | (seed << 8) | i), r_bits=R_BITS, window_bits=0))
+    # the autotuner runs on one key per size; its choices apply to the others
+    tune = {}
+    for first in (keys[0], keys[8]):
+        rep = first.autotune()
+        tune[first.key_bits] = rep
+        for k in keys:
+            if k.key_bits == first.key_bits and k is not first:
+                if k.table_window != first.table_window:
+                    k.table_window = first.table_window
+                k.dec_lanes = first.dec_lanes
     lens = log_uniform_lengths(4, seed, 1, 16, 256, 262144)
     jobs = []  # (op, key, inputs)
     total = 0
@@ -217,6 +229,7 @@ def run_config4(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
         "batch_launches": st["launches"],
         "work_units": st["units"],
         "bit_exact_vs_plain": bool(same),
+        "autotune": tune,
     }
 
 

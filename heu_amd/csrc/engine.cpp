@@ -335,7 +335,7 @@ int upload_meta(heu_ctx* ctx, Scratch& sc, size_t off, const void* host, size_t 
 
 // Generic segmented reduction.  dot: weights given (device), else plain product.
 int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t* offsets, size_t segments,
-              uint32_t* out) {
+              uint32_t* out, const int64_t* idx = nullptr) {
   heu_ctx* ctx = key->ctx;
   const Ops* o = key->ops;
   const ModArgs mod = key->mod_n2();
@@ -389,9 +389,9 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
   // level 0
   if (dot)
     LAUNCH(ctx, "reduce_dot0", o->reduce_dot0(st, mod, ct, w, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off),
-                   sc.at<uint32_t>(lv[0].neg_off), sc.at<uint32_t>(tab_off)));
+                   sc.at<uint32_t>(lv[0].neg_off), sc.at<uint32_t>(tab_off), idx));
   else
-    LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, true, ct, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off)));
+    LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, true, ct, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off), idx));
   for (size_t i = 1; i < lv.size(); ++i) {
     if (dot) {
       LAUNCH(ctx, "reduce_mont", o->reduce_mont(st, mod, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off), (long)lv[i].ch.size(),
@@ -400,7 +400,7 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
                      sc.at<uint32_t>(lv[i].neg_off)));
     } else {
       LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, false, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off),
-                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off)));
+                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off), nullptr));
     }
   }
   // the last level has one chunk per segment, in segment order
@@ -867,6 +867,41 @@ int heu_dot_i64(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_
   KEY_PROLOGUE(key);
   if (!offsets || !out || !w) return fail(HEU_EINVAL, "null argument");
   return segmented(key, ct, w, offsets, segments, out);
+}
+
+int heu_sum_gather(heu_key* key, const uint32_t* ct, const int64_t* idx, const int64_t* offsets, size_t segments,
+                   uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (!offsets || !out || !idx || (!ct && segments && offsets[segments] > offsets[0]))
+    return fail(HEU_EINVAL, "null argument");
+  return segmented(key, ct, nullptr, offsets, segments, out, idx);
+}
+
+int heu_dot_gather(heu_key* key, const uint32_t* ct, const int64_t* idx, const int64_t* w, const int64_t* offsets,
+                   size_t segments, uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (!offsets || !out || !idx || (segments && offsets[segments] > offsets[0] && (!ct || !w)))
+    return fail(HEU_EINVAL, "null argument");
+  return segmented(key, ct, w, offsets, segments, out, idx);
+}
+
+int heu_prefix_mul(heu_key* key, const uint32_t* ct, const int64_t* offsets, size_t segments, uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (segments == 0) return HEU_OK;
+  if (!ct || !offsets || !out) return fail(HEU_EINVAL, "null argument");
+  std::vector<Chunk> segs(segments);
+  for (size_t s = 0; s < segments; ++s) {
+    if (offsets[s + 1] < offsets[s]) return fail(HEU_EINVAL, "offsets must be non-decreasing");
+    segs[s] = Chunk{offsets[s], offsets[s + 1] - offsets[s]};
+  }
+  Scratch sc(ctx);
+  size_t off = sc.add(segments * sizeof(Chunk));
+  int rc = sc.commit();
+  if (rc) return rc;
+  if ((rc = upload_meta(ctx, sc, off, segs.data(), segments * sizeof(Chunk)))) return rc;
+  LAUNCH(ctx, "prefix_mul", o->prefix_mul(ctx->stream, key->mod_n2(), ct, sc.at<Chunk>(off), (long)segments, out));
+  HIP_TRY(hipGetLastError());
+  return HEU_OK;
 }
 
 static int mont_convert(heu_key* key, const uint32_t* ct, size_t count, uint32_t word_bits, uint32_t* out,

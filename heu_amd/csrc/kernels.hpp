@@ -62,8 +62,8 @@ enum WorkOp : int {
   kWorkAdd,          // a, b: ct, out: ct
   kWorkAddPt,        // a: ct, b: int64 pt, out: ct
   kWorkMul,          // a: ct, b: int64 scalar, out: ct (|k| power), neg: sign flags
-  kWorkDot0,         // a: ct, b: int64 weights, c: chunks, out/out2: pos/neg partials (count = chunks)
-  kWorkSum0,         // a: ct, c: chunks, out: partials (count = chunks)
+  kWorkDot0,         // a: ct, b: int64 weights, c: chunks, d: index list or null, out/out2: pos/neg partials
+  kWorkSum0,         // a: ct, c: chunks, d: index list or null, out: partials (count = chunks)
   kWorkDecHalf       // a: ct, c: window schedule (aux ops), out: this half's outputs; pub.n2 = p^2 or q^2
 };
 
@@ -75,6 +75,7 @@ struct WorkItem {
   const void* a;
   const void* b;
   const void* c;
+  const void* d;  // kWorkDot0 / kWorkSum0: optional int64 index list (gathered sums)
   void* out;
   void* out2;
   u8* neg;
@@ -93,9 +94,11 @@ typedef void (*AddPtFn)(hipStream_t, const PubArgs&, const uint32_t*, const int6
 typedef void (*PowFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, uint32_t*, u8*, uint32_t*);
 typedef void (*PowWordsFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, const u8*, int, long,
                            uint32_t*, u8*, uint32_t*);
-typedef void (*ReduceSumFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const Chunk*, long, uint32_t*);
+typedef void (*ReduceSumFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const Chunk*, long, uint32_t*,
+                            const int64_t*);
 typedef void (*ReduceDot0Fn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, const Chunk*, long,
-                             uint32_t*, uint32_t*, uint32_t*);
+                             uint32_t*, uint32_t*, uint32_t*, const int64_t*);
+typedef void (*PrefixMulFn)(hipStream_t, const ModArgs&, const uint32_t*, const Chunk*, long, uint32_t*);
 typedef void (*ReduceMontFn)(hipStream_t, const ModArgs&, const uint32_t*, const Chunk*, long, uint32_t*);
 typedef void (*SegFinishFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, int, uint32_t*);
 typedef void (*MontTimesZFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
@@ -154,6 +157,7 @@ struct Ops {
   InvDownFn inv_down;
   InvSingleFn inv_single;
   MulConstFn mul_const;
+  PrefixMulFn prefix_mul;
   CombineFn combine;
   WorklistShape wl_n;             // n^2 ops: encrypt, randomize, add, add_pt, mul, dot0, sum0
   DecShape dec[kDecShapes];       // CRT decryption halves mod p^2 / q^2; [0] is the default

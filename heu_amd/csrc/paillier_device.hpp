@@ -736,9 +736,13 @@ __global__ __launch_bounds__(kBlock) void k_dec_combine(CombArgs C, const u32* _
 //    positive or negative accumulator by the sign of w (w = 0 contributes 1).
 // Partials are R29 interleaved.
 // ---------------------------------------------------------------------------
+// Chunk positions j address element idx[j] when an index list is given
+// (gathered sums: selections, bucket sums), else element j.
+__device__ __forceinline__ long at_pos(const i64* __restrict__ idx, long j) { return idx ? (long)idx[j] : j; }
+
 template <int L, int K, int W, bool ZIN>
 __device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const u32* __restrict__ in, const Chunk ch,
-                                                u32* __restrict__ part) {
+                                                u32* __restrict__ part, const i64* __restrict__ idx = nullptr) {
   constexpr int NL = L * K;
   u32 x[K];
   if (ch.len == 0) {
@@ -746,11 +750,11 @@ __device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const u32* __re
     store_r29<L, K>(part, x, g.lid);
     return;
   }
-  if constexpr (ZIN) g.load_words(x, in + ch.start * W, W);
+  if constexpr (ZIN) g.load_words(x, in + at_pos(idx, ch.start) * W, W);
   else load_r29<L, K>(x, in + ch.start * NL, g.lid);
   g.set(g.A, x);
   for (long i = ch.start + 1; i < ch.start + ch.len; ++i) {
-    if constexpr (ZIN) g.load_words(x, in + i * W, W);
+    if constexpr (ZIN) g.load_words(x, in + at_pos(idx, i) * W, W);
     else load_r29<L, K>(x, in + i * NL, g.lid);
     g.mul(x);
   }
@@ -760,18 +764,20 @@ __device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const u32* __re
 template <int L, int K, int W, bool ZIN>
 __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_sum(ModArgs mod, const u32* __restrict__ in,
                                                                              const Chunk* __restrict__ chunks,
-                                                                             long nchunks, u32* __restrict__ part) {
+                                                                             long nchunks, u32* __restrict__ part,
+                                                                             const i64* __restrict__ idx) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e < nchunks) reduce_sum_body<L, K, W, ZIN>(g, in, chunks[e], part + e * NL);
+  if (e < nchunks) reduce_sum_body<L, K, W, ZIN>(g, in, chunks[e], part + e * NL, idx);
 }
 
 template <int L, int K, int W>
 __device__ __forceinline__ void reduce_dot0_body(Grp<L, K, W>& g, const ModArgs& mod, const u32* __restrict__ in,
                                                  const i64* __restrict__ wts, const Chunk ch, u32* __restrict__ pp,
-                                                 u32* __restrict__ pn, u32* __restrict__ T) {
+                                                 u32* __restrict__ pn, u32* __restrict__ T,
+                                                 const i64* __restrict__ idx = nullptr) {
   {
     u32 one[K];
     load_r29<L, K>(one, mod.one, g.lid);
@@ -783,7 +789,7 @@ __device__ __forceinline__ void reduce_dot0_body(Grp<L, K, W>& g, const ModArgs&
     if (w == 0) continue;
     const u64 mag = w < 0 ? (u64)0 - (u64)w : (u64)w;
     u32 x[K];
-    g.load_words(x, in + i * W, W);
+    g.load_words(x, in + at_pos(idx, i) * W, W);
     g.set(g.A, x);
     g.mul_global(mod.r2);  // c R
     if (mag != 1) pow_u64(g, mod, mag, T);
@@ -801,14 +807,15 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_dot0(
                                                                               const i64* __restrict__ wts,
                                                                               const Chunk* __restrict__ chunks,
                                                                               long nchunks, u32* __restrict__ part_pos,
-                                                                              u32* __restrict__ part_neg, u32* tab) {
+                                                                              u32* __restrict__ part_neg, u32* tab,
+                                                                              const i64* __restrict__ idx) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, mod, e);
   if (e < nchunks)
     reduce_dot0_body(g, mod, in, wts, chunks[e], part_pos + e * NL, part_neg + e * NL,
-                     tab + e * (long)(NL << kPowWin));
+                     tab + e * (long)(NL << kPowWin), idx);
 }
 
 // Upper levels of a dot: products of Montgomery-form partials (stay in M-space).
@@ -874,6 +881,34 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mont_times_z
   g.set_global(g.A, xm + e * NL);
   g.mul(y);
   g.store_canonical(out + e * W, W);
+}
+
+// ---------------------------------------------------------------------------
+// Inclusive prefix products within segments (cumulative bucket sums,
+// numpy/evaluator.cc:450-456): out[j] = in[start] * ... * in[j] mod m for j
+// in each segment, Z-space in and out; one group walks one segment.  Each step
+// is (acc * x) R^-1 * R^2 R^-1.
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_prefix_mul(ModArgs mod, const u32* __restrict__ in,
+                                                                             const Chunk* __restrict__ segs, long nseg,
+                                                                             u32* __restrict__ out) {
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= nseg) return;
+  const Chunk sg = segs[e];
+  if (sg.len <= 0) return;
+  u32 x[K];
+  g.load_words(x, in + sg.start * W, W);
+  g.set(g.A, x);
+  g.store_canonical(out + sg.start * W, W);
+  for (long j = sg.start + 1; j < sg.start + sg.len; ++j) {
+    g.load_words(x, in + j * W, W);
+    g.mul(x);
+    g.mul_global(mod.r2);
+    g.store_canonical(out + j * W, W);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1254,11 +1289,11 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_worklist(con
           case kWorkDot0:
             reduce_dot0_body(g, it.pub.n2, static_cast<const u32*>(it.a), static_cast<const i64*>(it.b),
                              static_cast<const Chunk*>(it.c)[e], static_cast<u32*>(it.out) + e * NL,
-                             static_cast<u32*>(it.out2) + e * NL, T);
+                             static_cast<u32*>(it.out2) + e * NL, T, static_cast<const i64*>(it.d));
             break;
           case kWorkSum0:
             reduce_sum_body<L, K, W, true>(g, static_cast<const u32*>(it.a), static_cast<const Chunk*>(it.c)[e],
-                                           static_cast<u32*>(it.out) + e * NL);
+                                           static_cast<u32*>(it.out) + e * NL, static_cast<const i64*>(it.d));
             break;
           default:
             break;
@@ -1330,13 +1365,16 @@ struct Launch {
     HEU_LAUNCH((k_pow_words<L, K, W>), n, L, st, mod, ct, kmag, kneg, kw, n, out, neg, tab);
   }
   static void reduce_sum(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const Chunk* chunks, long nch,
-                         u32* part) {
-    if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part);
-    else HEU_LAUNCH((k_reduce_sum<L, K, W, false>), nch, L, st, mod, in, chunks, nch, part);
+                         u32* part, const i64* idx) {
+    if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part, idx);
+    else HEU_LAUNCH((k_reduce_sum<L, K, W, false>), nch, L, st, mod, in, chunks, nch, part, nullptr);
   }
   static void reduce_dot0(hipStream_t st, const ModArgs& mod, const u32* in, const i64* w, const Chunk* chunks,
-                          long nch, u32* pp, u32* pn, u32* tab) {
-    HEU_LAUNCH((k_reduce_dot0<L, K, W>), nch, L, st, mod, in, w, chunks, nch, pp, pn, tab);
+                          long nch, u32* pp, u32* pn, u32* tab, const i64* idx) {
+    HEU_LAUNCH((k_reduce_dot0<L, K, W>), nch, L, st, mod, in, w, chunks, nch, pp, pn, tab, idx);
+  }
+  static void prefix_mul(hipStream_t st, const ModArgs& mod, const u32* in, const Chunk* segs, long nseg, u32* out) {
+    HEU_LAUNCH((k_prefix_mul<L, K, W>), nseg, L, st, mod, in, segs, nseg, out);
   }
   static void reduce_mont(hipStream_t st, const ModArgs& mod, const u32* in, const Chunk* chunks, long nch,
                           u32* part) {
@@ -1449,6 +1487,7 @@ static Ops make_ops(const WorklistShape& wl_n, const DecShape& dec0, const DecSh
   o.inv_down = Launch<SN>::inv_down;
   o.inv_single = Launch<SN>::inv_single;
   o.mul_const = Launch<SN>::mul_const;
+  o.prefix_mul = Launch<SN>::prefix_mul;
   o.combine = launch_combine<PW>;
   o.wl_n = wl_n;
   o.dec[0] = dec0;

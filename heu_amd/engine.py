@@ -362,6 +362,32 @@ class Key:
         check(lib().heu_dot_i64(self._h, _ptr(ct), _ptr(w), _ptr(offs), segs, _ptr(out)))
         return out
 
+    def sum_gather(self, ct, idx, offsets) -> torch.Tensor:
+        """out[s] = prod ct[idx[j]] over positions j of segment s."""
+        ct, idx = self._dev(ct, torch.int32), self._dev(idx, torch.int64)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        segs = len(offs) - 1
+        out = self._ct_out(segs)
+        check(lib().heu_sum_gather(self._h, _ptr(ct), _ptr(idx), _ptr(offs), segs, _ptr(out)))
+        return out
+
+    def dot_gather(self, ct, idx, w, offsets) -> torch.Tensor:
+        """out[s] = prod ct[idx[j]]^w[j] over positions j of segment s."""
+        ct, idx, w = self._dev(ct, torch.int32), self._dev(idx, torch.int64), self._dev(w, torch.int64)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        segs = len(offs) - 1
+        out = self._ct_out(segs)
+        check(lib().heu_dot_gather(self._h, _ptr(ct), _ptr(idx), _ptr(w), _ptr(offs), segs, _ptr(out)))
+        return out
+
+    def prefix_mul(self, ct, offsets) -> torch.Tensor:
+        """Inclusive prefix products within each segment."""
+        ct = self._dev(ct, torch.int32)
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        out = self._ct_out(self._rows(ct))
+        check(lib().heu_prefix_mul(self._h, _ptr(ct), _ptr(offs), len(offs) - 1, _ptr(out)))
+        return out
+
     def to_mont(self, ct, word_bits: int = 64) -> torch.Tensor:
         ct = self._dev(ct, torch.int32)
         n = self._rows(ct)

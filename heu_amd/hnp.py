@@ -1,0 +1,377 @@
+"""numpy-style arrays of Paillier ciphertexts on the device: the HeMatrix
+surface of HEU's `heu.numpy` (heu/pylib/numpy_binding/bind_numpy.cc:321-510,
+heu/library/numpy/evaluator.h:24-92) for the Paillier path, over the engine.
+
+  kit = hnp.setup(2048)                        # keys resident on the GPU
+  x = kit.encryptor().encrypt(np_int64_array)  # CiphertextArray (1-D or 2-D)
+  y = kit.evaluator().add(x, other)            # numpy broadcasting, ct/pt mixes
+  s = kit.evaluator().select_sum(x, (slice(None), 0))
+  h = kit.evaluator().feature_wise_bucket_sum(x, subgroup_map, order_map, 16)
+  z = kit.decryptor().decrypt(y)               # numpy int64 (object if wider)
+
+Plaintext arrays are numpy integer arrays (the reference wraps them in
+PlaintextArray).  Ciphertext arrays keep C order on the device; every
+reduction, selection and bucket sum runs as one gathered engine call
+(`heu_sum_gather` / `heu_dot_gather`), so no gathered copy is made.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import torch
+
+from . import engine as E
+
+
+class CiphertextArray:
+    """A 0-, 1- or 2-D array of ciphertexts of one key, `data` being a device
+    tensor of shape (size, ct_words), C order."""
+
+    def __init__(self, key: E.Key, data: torch.Tensor, shape):
+        self.key = key
+        self.data = data
+        self.shape = tuple(int(s) for s in shape)
+        if data.shape[0] != int(np.prod(self.shape, dtype=np.int64)):
+            raise ValueError(f"{data.shape[0]} ciphertexts do not fill shape {self.shape}")
+
+    @property
+    def ndim(self) -> int:
+        return len(self.shape)
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    def rows(self) -> int:
+        return self.shape[0] if self.ndim else 1
+
+    def cols(self) -> int:
+        return self.shape[1] if self.ndim == 2 else 1
+
+    def _flat_index(self, key) -> np.ndarray:
+        """Indices into `data` selected by a numpy-style key (ints, slices,
+        integer or boolean arrays, tuples of those)."""
+        grid = np.arange(self.size, dtype=np.int64).reshape(self.shape)
+        return np.asarray(grid[key], dtype=np.int64)
+
+    def __getitem__(self, key) -> "CiphertextArray":
+        sel = self._flat_index(key)
+        idx = torch.from_numpy(np.ascontiguousarray(sel.ravel())).to(self.data.device)
+        return CiphertextArray(self.key, self.data.index_select(0, idx), sel.shape)
+
+    def transpose(self) -> "CiphertextArray":
+        if self.ndim < 2:
+            return self
+        return self._take(np.arange(self.size, dtype=np.int64).reshape(self.shape).T)
+
+    @property
+    def T(self) -> "CiphertextArray":
+        return self.transpose()
+
+    def _take(self, grid: np.ndarray) -> "CiphertextArray":
+        idx = torch.from_numpy(np.ascontiguousarray(grid.ravel())).to(self.data.device)
+        return CiphertextArray(self.key, self.data.index_select(0, idx), grid.shape)
+
+    def reshape(self, *shape) -> "CiphertextArray":
+        shape = shape[0] if len(shape) == 1 and isinstance(shape[0], (tuple, list)) else shape
+        shape = np.empty(self.size, dtype=np.int8).reshape(shape).shape
+        return CiphertextArray(self.key, self.data, shape)
+
+    def __len__(self) -> int:
+        return self.shape[0] if self.ndim else 1
+
+    def __repr__(self) -> str:
+        return f"CiphertextArray(shape={self.shape}, key_bits={self.key.key_bits})"
+
+
+def _is_ct(x) -> bool:
+    return isinstance(x, CiphertextArray)
+
+
+def _pt(x) -> np.ndarray:
+    a = np.asarray(x)
+    if a.dtype == object or not np.issubdtype(a.dtype, np.integer):
+        raise TypeError("plaintext arrays must hold integers that fit int64")
+    return a.astype(np.int64)
+
+
+class Encryptor:
+    """Encrypts numpy integer arrays (numpy/encryptor.cc:19-66)."""
+
+    def __init__(self, kit: "HeKit"):
+        self.kit = kit
+
+    def _r(self, count: int, seed=None) -> np.ndarray:
+        key = self.kit.key
+        rw = (key.r_bits + 31) // 32
+        if seed is not None:
+            return E.seeded_r(seed, 1, count, key.r_bits)
+        raw = np.frombuffer(os.urandom(count * rw * 4), dtype=np.uint32).reshape(count, rw).copy()
+        if key.r_bits % 32:
+            raw[:, -1] &= np.uint32((1 << (key.r_bits % 32)) - 1)
+        return raw
+
+    def encrypt(self, pts, seed=None) -> CiphertextArray:
+        a = _pt(pts)
+        flat = np.ascontiguousarray(a.ravel())
+        if flat.size == 0:
+            return CiphertextArray(self.kit.key, self.kit.key._ct_out(0), a.shape)
+        ct = self.kit.key.encrypt_i64(flat, self._r(flat.size, seed))
+        return CiphertextArray(self.kit.key, ct, a.shape)
+
+
+class Decryptor:
+    """Decrypts to numpy arrays: int64 when every value fits, else Python ints
+    (numpy/decryptor.cc:19-99)."""
+
+    def __init__(self, kit: "HeKit"):
+        self.kit = kit
+
+    def decrypt(self, ct: CiphertextArray):
+        key = self.kit.key
+        if ct.size == 0:
+            return np.zeros(ct.shape, dtype=np.int64)
+        out, ovf = key.decrypt(ct.data, out_words=2, overflow=True)
+        vals = out.cpu().numpy().view(np.int64).reshape(ct.shape)
+        if not bool(ovf.any()):
+            return vals
+        wide = key.decrypt(ct.data, out_words=key.n_words).cpu().numpy().view(np.uint32)
+        objs = np.array([E.twos_to_int(row) for row in wide], dtype=object)
+        return objs.reshape(ct.shape)
+
+
+class Evaluator:
+    """Elementwise ops with numpy broadcasting, matrix products and sums
+    (numpy/evaluator.h:24-92, pylib extension_functions.cc)."""
+
+    def __init__(self, kit: "HeKit"):
+        self.kit = kit
+
+    # ------------------------------------------------------------ helpers
+    def _bcast(self, x, y):
+        """Broadcasts two operands; ciphertext operands become flat index
+        arrays into their data, plaintext operands flat int64 arrays."""
+        sx = x.shape if _is_ct(x) else np.shape(x)
+        sy = y.shape if _is_ct(y) else np.shape(y)
+        shape = np.broadcast_shapes(sx, sy)
+
+        def expand(v, s):
+            if _is_ct(v):
+                grid = np.broadcast_to(np.arange(v.size, dtype=np.int64).reshape(s), shape)
+                return np.ascontiguousarray(grid.ravel())
+            return np.ascontiguousarray(np.broadcast_to(_pt(v), shape).ravel())
+
+        return shape, expand(x, sx), expand(y, sy)
+
+    def _rows_of(self, ct: CiphertextArray, idx: np.ndarray) -> torch.Tensor:
+        if idx.size == ct.size and np.array_equal(idx, np.arange(ct.size)):
+            return ct.data
+        return ct.data.index_select(0, torch.from_numpy(idx).to(ct.data.device))
+
+    def _key(self, *xs) -> E.Key:
+        keys = {id(x.key): x.key for x in xs if _is_ct(x)}
+        if len(keys) != 1:
+            raise ValueError("ciphertext operands must belong to one key")
+        return next(iter(keys.values()))
+
+    # ------------------------------------------------------------ elementwise
+    def add(self, x, y):
+        if not _is_ct(x) and not _is_ct(y):
+            return np.add(_pt(x), _pt(y))
+        key = self._key(x, y)
+        shape, ix, iy = self._bcast(x, y)
+        if ix.size == 0:
+            return CiphertextArray(key, key._ct_out(0), shape)
+        if _is_ct(x) and _is_ct(y):
+            out = key.add(self._rows_of(x, ix), self._rows_of(y, iy))
+        elif _is_ct(x):
+            out = key.add_i64(self._rows_of(x, ix), iy)
+        else:
+            out = key.add_i64(self._rows_of(y, iy), ix)
+        return CiphertextArray(key, out, shape)
+
+    def sub(self, x, y):
+        if not _is_ct(x) and not _is_ct(y):
+            return np.subtract(_pt(x), _pt(y))
+        key = self._key(x, y)
+        shape, ix, iy = self._bcast(x, y)
+        if ix.size == 0:
+            return CiphertextArray(key, key._ct_out(0), shape)
+        if _is_ct(x) and _is_ct(y):  # evaluator.cc:66-68
+            out = key.sub(self._rows_of(x, ix), self._rows_of(y, iy))
+        elif _is_ct(x):  # Add(ct, -p), evaluator.cc:70-72
+            neg = [-int(v) for v in iy]
+            if min(iy, default=0) == np.iinfo(np.int64).min:
+                gm = key.encrypt(neg, np.zeros((len(neg), 1), np.uint32))
+                out = key.add(self._rows_of(x, ix), gm)
+            else:
+                out = key.add_i64(self._rows_of(x, ix), -iy)
+        else:  # Add(Negate(ct), p), evaluator.cc:78-80
+            out = key.add_i64(key.negate(self._rows_of(y, iy)), ix)
+        return CiphertextArray(key, out, shape)
+
+    def mul(self, x, y):
+        if not _is_ct(x) and not _is_ct(y):
+            return np.multiply(_pt(x), _pt(y))
+        if _is_ct(x) and _is_ct(y):
+            raise TypeError("ciphertext x ciphertext is not defined for Paillier")
+        key = self._key(x, y)
+        shape, ix, iy = self._bcast(x, y)
+        if ix.size == 0:
+            return CiphertextArray(key, key._ct_out(0), shape)
+        ct, idx, k = (x, ix, iy) if _is_ct(x) else (y, iy, ix)
+        return CiphertextArray(key, key.mul_i64(self._rows_of(ct, idx), k), shape)
+
+    # ------------------------------------------------------------ products
+    def matmul(self, x, y):
+        """x @ y for (ct, pt) or (pt, ct) operands, 1-D or 2-D (numpy/evaluator.cc:209-292):
+        out[i, j] = sum_l x[i, l] * y[l, j], one gathered dot per output."""
+        if not _is_ct(x) and not _is_ct(y):
+            return np.matmul(_pt(x), _pt(y))
+        if _is_ct(x) and _is_ct(y):
+            raise TypeError("ciphertext @ ciphertext is not defined for Paillier")
+        key = self._key(x, y)
+        xs = x.shape if _is_ct(x) else np.shape(x)
+        ys = y.shape if _is_ct(y) else np.shape(y)
+        x2 = xs if len(xs) == 2 else (1, xs[0])
+        y2 = ys if len(ys) == 2 else (ys[0], 1)
+        n, k = x2
+        k2, m = y2
+        if k != k2:
+            raise ValueError(f"matmul: shapes {xs} and {ys} not aligned")
+        out_shape = tuple(([n] if len(xs) == 2 else []) + ([m] if len(ys) == 2 else []))
+        i, j, l = np.meshgrid(np.arange(n), np.arange(m), np.arange(k), indexing="ij")
+        if _is_ct(x):
+            idx = (i * k + l).ravel()
+            w = _pt(y).reshape(k, m)[l, j].ravel()
+            ct = x
+        else:
+            idx = (l * m + j).ravel()
+            w = _pt(x).reshape(n, k)[i, l].ravel()
+            ct = y
+        offs = np.arange(0, n * m * k + 1, k, dtype=np.int64)
+        out = key.dot_gather(ct.data, np.ascontiguousarray(idx, dtype=np.int64), np.ascontiguousarray(w), offs)
+        return CiphertextArray(key, out, out_shape)
+
+    # ------------------------------------------------------------ reductions
+    def sum(self, x):
+        """Sum of every element (numpy/evaluator.cc:355-373); a 0-D array."""
+        if not _is_ct(x):
+            return np.sum(_pt(x))
+        out = x.key.sum(x.data, np.array([0, x.size], np.int64)) if x.size else None
+        if out is None:
+            out = x.key.sum(x.key._ct_out(0), np.array([0, 0], np.int64))
+        return CiphertextArray(x.key, out, ())
+
+    def select_sum(self, x, key):
+        """x[key].sum() without the copy (extension_functions.cc:29-57); an
+        empty selection gives an encryption of 0."""
+        if not _is_ct(x):
+            return np.sum(_pt(x)[key])
+        return self.batch_select_sum(x, [key]).reshape(())
+
+    def batch_select_sum(self, x, keys):
+        """[x[k].sum() for k in keys] as one 1-D array (extension_functions.cc:59-87)."""
+        if not _is_ct(x):
+            return np.array([np.sum(_pt(x)[k]) for k in keys], dtype=np.int64)
+        sels = [x._flat_index(k).ravel() for k in keys]
+        offs = np.concatenate([[0], np.cumsum([len(s) for s in sels])]).astype(np.int64)
+        idx = np.concatenate(sels).astype(np.int64) if sels else np.zeros(0, np.int64)
+        if idx.size == 0:
+            idx = np.zeros(1, np.int64)  # a valid pointer; every segment is empty
+        out = x.key.sum_gather(x.data, idx, offs)
+        return CiphertextArray(x.key, out, (len(keys),))
+
+    def feature_wise_bucket_sum(self, x, subgroup_map, order_map, bucket_num: int, cumsum: bool = False):
+        """Per feature f, bucket b and column c: the sum of x[i, c] over the rows
+        i of the subgroup with order_map[i, f] == b (cumulated over b with
+        cumsum); shape (bucket_num * features, x.cols())
+        (numpy/evaluator.cc:378-466, extension_functions.cc:89-107)."""
+        return self.batch_feature_wise_bucket_sum(x, [subgroup_map], order_map, bucket_num, cumsum)[0]
+
+    def batch_feature_wise_bucket_sum(self, x, subgroup_maps, order_map, bucket_num: int, cumsum: bool = False):
+        if not _is_ct(x):
+            return [self._bucket_sum_plain(_pt(x), np.asarray(sm), np.asarray(order_map), bucket_num, cumsum)
+                    for sm in subgroup_maps]
+        om = np.asarray(order_map)
+        if om.ndim != 2 or om.shape[0] != x.rows():
+            raise ValueError("order map and x should have the same number of rows")
+        rows, cols, feats, B = x.rows(), x.cols(), om.shape[1], int(bucket_num)
+        if om.size and (om.min() < 0 or om.max() >= B):
+            raise ValueError("order map entries must be in [0, bucket_num)")
+        idx_parts, lens = [], []
+        for sm in subgroup_maps:
+            sub = np.nonzero(np.asarray(sm).ravel() > 0)[0]
+            for f in range(feats):
+                b_of = om[sub, f].astype(np.int64)
+                order = np.argsort(b_of, kind="stable")
+                grouped = sub[order]
+                counts = np.bincount(b_of, minlength=B)
+                starts = np.concatenate([[0], np.cumsum(counts)])
+                for c in range(cols):  # segments in (group, f, c, b) order
+                    elem = grouped * cols + c
+                    for b in range(B):
+                        idx_parts.append(elem[starts[b]:starts[b + 1]])
+                        lens.append(int(counts[b]))
+        offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        idx = np.concatenate(idx_parts).astype(np.int64) if idx_parts else np.zeros(0, np.int64)
+        if idx.size == 0:
+            idx = np.zeros(1, np.int64)
+        sums = x.key.sum_gather(x.data, idx, offs)  # (G * F * C * B, W)
+        if cumsum:
+            seg = np.arange(0, sums.shape[0] + 1, B, dtype=np.int64)
+            sums = x.key.prefix_mul(sums, seg)
+        G = len(subgroup_maps)
+        # (g, f, c, b) -> (g, f, b, c): rows f * B + b, columns c
+        perm = np.arange(G * feats * cols * B).reshape(G, feats, cols, B).transpose(0, 1, 3, 2).ravel()
+        sums = sums.index_select(0, torch.from_numpy(perm).to(sums.device))
+        per = feats * B * cols
+        return [CiphertextArray(x.key, sums[g * per:(g + 1) * per], (feats * B, cols)) for g in range(G)]
+
+    @staticmethod
+    def _bucket_sum_plain(x, sm, om, B, cumsum):
+        x2 = x.reshape(x.shape[0], -1)
+        sub = np.nonzero(sm.ravel() > 0)[0]
+        out = np.zeros((B * om.shape[1], x2.shape[1]), dtype=np.int64)
+        for f in range(om.shape[1]):
+            for i in sub:
+                out[f * B + om[i, f]] += x2[i]
+            if cumsum:
+                out[f * B:(f + 1) * B] = np.cumsum(out[f * B:(f + 1) * B], axis=0)
+        return out
+
+
+class HeKit:
+    """Keys resident on the device plus the three actors (numpy HeKit,
+    bind_numpy.cc:242-293)."""
+
+    def __init__(self, key: E.Key):
+        self.key = key
+        self._enc, self._dec, self._ev = Encryptor(self), Decryptor(self), Evaluator(self)
+
+    def encryptor(self) -> Encryptor:
+        return self._enc
+
+    def decryptor(self) -> Decryptor:
+        return self._dec
+
+    def evaluator(self) -> Evaluator:
+        return self._ev
+
+    def public_key(self):
+        return self.key.n, self.key.hs
+
+    def array(self, ndarray) -> np.ndarray:
+        """A plaintext array (numpy int64)."""
+        return _pt(ndarray)
+
+
+def setup(key_bits: int = 2048, seed: int | None = None, ctx: E.Context | None = None,
+          r_bits: int | None = None, window_bits: int = 0) -> HeKit:
+    """Generates a key (seeded or from the OS RNG) and makes it resident.
+    r_bits defaults to key_bits / 2 like the reference's GetRn (encryptor.cc:27)."""
+    ctx = ctx or E.Context(0)
+    kd = E.keygen(key_bits, seed)
+    key = E.Key.from_keygen(ctx, kd, r_bits=r_bits or key_bits // 2, window_bits=window_bits)
+    return HeKit(key)

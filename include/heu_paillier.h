@@ -148,6 +148,23 @@ int heu_sum(heu_key *key, const uint32_t *ct, const int64_t *offsets, size_t seg
  * prod ct[i]^w[i] over the segment.  w: device array, one weight per ct. */
 int heu_dot_i64(heu_key *key, const uint32_t *ct, const int64_t *w, const int64_t *offsets, size_t segments,
                 uint32_t *out);
+/* Gathered forms for selections, bucket sums and matrix products
+ * (numpy/evaluator.h:57-70 SelectSum, numpy/evaluator.cc:378-466
+ * FeatureWiseBucketSum, :209-292 MatMul; pylib/numpy_binding/
+ * extension_functions.cc:29-87 SelectSum / BatchSelectSum): positions j of
+ * segment s, j in [offsets[s], offsets[s+1]) (HOST offsets), address
+ * ciphertext idx[j] (device int64 indices into ct):
+ *   heu_sum_gather: out[s] = prod ct[idx[j]]            (1 for an empty segment)
+ *   heu_dot_gather: out[s] = prod ct[idx[j]]^w[j]       (w: device int64, per position) */
+int heu_sum_gather(heu_key *key, const uint32_t *ct, const int64_t *idx, const int64_t *offsets, size_t segments,
+                   uint32_t *out);
+int heu_dot_gather(heu_key *key, const uint32_t *ct, const int64_t *idx, const int64_t *w, const int64_t *offsets,
+                   size_t segments, uint32_t *out);
+/* Inclusive prefix products within segments (the cumsum of
+ * numpy/evaluator.cc:450-456): out[j] = ct[offsets[s]] * ... * ct[j] mod n^2
+ * for j in segment s; out may alias ct. */
+int heu_prefix_mul(heu_key *key, const uint32_t *ct, const int64_t *offsets, size_t segments, uint32_t *out);
+
 /* Montgomery-form export / import for R = 2^(word_bits*ceil(bits(n^2)/word_bits))
  * (MontgomerySpace::MapIntoMSpace / MapBackToZSpace, encryptor.cc:46,
  * decryptor.cc:36). */
