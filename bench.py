@@ -112,7 +112,7 @@ def cpu_baseline(kd, m, r, k, budget_s):
     run("dec", lambda n: key.decrypt(ensure(n), out_words=2, threads=threads), threads)
     chain = 1.0 / sum(1.0 / rates[x] for x in ("enc", "add", "ctpt", "dec"))
     _ = n_ct
-    return {
+    return ctb, {
         "value": chain,
         "unit": "elements/s (enc+add+ct*pt+dec)",
         "cores": threads,
@@ -222,10 +222,14 @@ def main():
 
     ms_per_step = elapsed / args.steps * 1e3
     value = N / (elapsed / args.steps)
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
         n_cpu = min(N, 1 << 16)
-        cpu = cpu_baseline(kd, m_h[:n_cpu], r_h[:n_cpu], k_h[:n_cpu], args.cpu_seconds)
+        ref_ct, cpu = cpu_baseline(kd, m_h[:n_cpu], r_h[:n_cpu], k_h[:n_cpu], args.cpu_seconds)
+        # the GPU's ciphertexts of the same inputs against the oracle's (paillier_z restated)
+        n_par = len(ref_ct)
+        gpu_ct = out[0][:n_par].cpu().numpy().view(np.uint32)
+        parity = {"elements": n_par, "encrypt_bit_exact": bool(np.array_equal(gpu_ct, ref_ct))}
     line = {
         "metric": METRIC,
         "value": value,
@@ -271,6 +275,7 @@ def main():
     }
     if cpu:
         line["cpu_baseline"]["speedup_gpu_over_cpu"] = value / cpu["value"]
+        line["parity_subset"] = parity
     extra = {}
     wanted = [c.strip() for c in args.configs.split(",") if c.strip()]
     if wanted:

@@ -366,3 +366,13 @@ def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0), per_rat
         "max_requests_per_batch": st["max_batch_requests"],
         "alone_equals_mix": check_same,
     }
+
+
+if __name__ == "__main__":
+    # python3 -m heu_amd.configs 3 [4 5]: runs the named configurations alone and prints their results
+    import json
+    import sys
+
+    ctx = E.Context(0)
+    out = {c: globals()[f"run_config{c}"](ctx) for c in (sys.argv[1:] or ["3"])}
+    print(json.dumps(out))

@@ -442,6 +442,7 @@ int heu_batch_create(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, uint32_
   batch->steps = std::move(p.steps);
   batch->launches = p.launches;
   batch->units = p.units;
+  ctx->refs += 1;  // under ctx->mu: the batch holds the context
   *out = batch.release();
   return HEU_OK;
 }
@@ -451,6 +452,7 @@ int heu_batch_run(heu_batch* batch) {
   heu_ctx* ctx = batch->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
+  (void)hipGetLastError();  // an error left by other code is not this run's
   if (!(batch->flags & HEU_BATCH_GRAPH)) {
     for (auto& s : batch->steps) s(ctx->stream);
     HIP_TRY(hipGetLastError());
@@ -475,13 +477,16 @@ int heu_batch_run(heu_batch* batch) {
 int heu_batch_destroy(heu_batch* batch) {
   if (!batch) return HEU_OK;
   heu_ctx* ctx = batch->ctx;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  if (batch->exec) (void)hipGraphExecDestroy(batch->exec);
-  if (batch->graph) (void)hipGraphDestroy(batch->graph);
-  if (batch->cap_stream) (void)hipStreamDestroy(batch->cap_stream);
-  delete batch;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (batch->exec) (void)hipGraphExecDestroy(batch->exec);
+    if (batch->graph) (void)hipGraphDestroy(batch->graph);
+    if (batch->cap_stream) (void)hipStreamDestroy(batch->cap_stream);
+    delete batch;
+  }
+  ctx_release(ctx);
   return HEU_OK;
 }
 

@@ -45,6 +45,32 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+static void ctx_free(heu_ctx* ctx) {
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->d_err) (void)hipFree(ctx->d_err);
+  ctx->scratch.alloc(0);
+  ctx->meta.alloc(0);
+  ctx->prof_clear();
+  for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+}
+
+void ctx_retain(heu_ctx* ctx) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ++ctx->refs;
+}
+
+void ctx_release(heu_ctx* ctx) {
+  bool free_now;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    free_now = --ctx->refs == 0 && ctx->closing;
+  }
+  if (free_now) ctx_free(ctx);
+}
+
 std::vector<long> inv_levels(long count) {
   std::vector<long> cnt{count};
   while (cnt.size() < 2 || cnt.back() > 1) cnt.push_back((cnt.back() + kInvChunk - 1) / kInvChunk);
@@ -450,15 +476,12 @@ int heu_ctx_create(int device, heu_ctx** out) {
 
 int heu_ctx_destroy(heu_ctx* ctx) {
   if (!ctx) return HEU_OK;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->d_err) (void)hipFree(ctx->d_err);
-  ctx->scratch.alloc(0);
-  ctx->meta.alloc(0);
-  ctx->prof_clear();
-  for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
-  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
-  delete ctx;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->closing = true;
+    if (ctx->refs > 0) return HEU_OK;  // freed with the last key / batch / server
+  }
+  ctx_free(ctx);
   return HEU_OK;
 }
 
@@ -624,16 +647,21 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
     key->c_qprime = (uint32_t)neg_inv_pow2(q, 32);
     key->has_sk = 1;
   }
+  ctx->refs += 1;  // under ctx->mu: the key holds the context
   *out = key.release();
   return HEU_OK;
 }
 
 int heu_key_destroy(heu_key* key) {
   if (!key) return HEU_OK;
-  std::lock_guard<std::mutex> lk(key->ctx->mu);
-  (void)hipSetDevice(key->ctx->device);
-  (void)hipStreamSynchronize(key->ctx->stream);
-  delete key;
+  heu_ctx* ctx = key->ctx;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();  // any stream (batches, servers) may still read the key
+    delete key;
+  }
+  ctx_release(ctx);
   return HEU_OK;
 }
 
@@ -680,12 +708,16 @@ int heu_key_get_param(const heu_key* key, int param, uint32_t* value) {
   }
 }
 
+// Every key entry point: validate, serialize on the context, select its
+// device, and drop any error another library left on this thread, so that the
+// hipGetLastError() checks after launches report this call's own launches.
 #define KEY_PROLOGUE(key)                          \
   int rc_ = check_key(key);                        \
   if (rc_) return rc_;                             \
   heu_ctx* ctx = (key)->ctx;                       \
   std::lock_guard<std::mutex> lk_(ctx->mu);        \
   HIP_TRY(hipSetDevice(ctx->device));              \
+  (void)hipGetLastError();                         \
   const Ops* o = (key)->ops;                       \
   (void)o;
 

@@ -73,6 +73,11 @@ struct PlanOut {
   std::vector<std::pair<size_t, std::vector<char>>> metas;
   std::vector<std::function<void(hipStream_t)>> steps;
 };
+// Context lifetime: objects created on a context hold a reference; the
+// context is freed by heu_ctx_destroy or by the last release after it.
+void ctx_retain(heu_ctx* ctx);
+void ctx_release(heu_ctx* ctx);
+
 // Checks the tasks and plans them (caller holds ctx->mu).
 int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, PlanOut* out);
 // Work-list units of `shape` that run at once on the whole device: resident
@@ -103,6 +108,10 @@ struct heu_ctx {
     const char* name;
     hipEvent_t start, stop;
   };
+  // keys, batches and servers created on this context: it is freed when the
+  // last of them goes after heu_ctx_destroy (whatever order callers release in)
+  int refs = 0;
+  bool closing = false;
   // persistent work-list launches: grid multiplier per [key size is 3072][decryption shape]
   // and the occupancy of each work-list kernel (resident workgroups per CU)
   uint32_t grid_mult[2][2] = {{1, 1}, {1, 1}};

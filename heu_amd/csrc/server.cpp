@@ -414,6 +414,7 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
         rc = HEU_EDEVICE;
       at += m.second.size();
     }
+    (void)hipGetLastError();  // only this batch's launches count below
     for (auto& s : real.steps) s(slot->stream);
     if (hipGetLastError() != hipSuccess) rc = HEU_EDEVICE;
   }
@@ -468,6 +469,7 @@ int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** o
     HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
   }
+  ctx_retain(ctx);  // the server holds the context
   heu_server* raw = srv.release();
   raw->worker = std::thread([raw] { raw->run(); });
   *out = raw;
@@ -550,7 +552,9 @@ int heu_server_destroy(heu_server* srv) {
     if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.pinned) (void)hipHostFree(s.pinned);
   }
+  heu_ctx* ctx = srv->ctx;
   delete srv;
+  ctx_release(ctx);
   return HEU_OK;
 }
 

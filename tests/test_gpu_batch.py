@@ -193,3 +193,27 @@ def test_batch_errors(eng, ctx, keys):
     b.decrypt(pub, gk.encrypt_i64(np.array([1], np.int64), eng.seeded_r(1, 1, 1, 64)))
     with pytest.raises(HeuError):
         b.plan()
+
+
+def test_context_outlives_its_objects(eng):
+    # heu_ctx_destroy before the key and batch made on it: they keep working,
+    # and the context is freed with the last of them (any release order)
+    from heu_amd._native import check, lib
+
+    c = eng.Context(0)
+    kd = eng.keygen(2048, 5)
+    k = eng.Key.from_keygen(c, kd, r_bits=64, window_bits=4)
+    m = np.array([1, -2, 3], np.int64)
+    b = eng.Batch(c)
+    ct = b.encrypt_i64(k, m, eng.seeded_r(1, 1, 3, 64))
+    b.plan()
+    h = c._h
+    check(lib().heu_ctx_destroy(h))  # deferred: the key and the batch hold it
+    c._h = None
+    b.run()
+    check(lib().heu_ctx_sync(h))
+    d = k.decrypt(ct, out_words=2)
+    check(lib().heu_ctx_sync(h))
+    assert np.array_equal(host(d).view(np.int64).ravel(), m)
+    b.close()
+    k.close()  # the last holder: frees the context

@@ -230,3 +230,28 @@ def test_errors(eng, ctx, k2048):
     with pytest.raises(HeuError):
         ctx.sync()
     ctx.sync()  # flags cleared
+
+
+def test_config1_workload(eng, oracle_mod, ctx):
+    # the CPU reference case: 2048-bit key from seed 0, two vectors of 4096
+    # int64-uniform plaintexts, r uniform in [1, 2^256) from the seed;
+    # encrypt both, ct + ct, decrypt -- every output bit-exact with the oracle
+    kd = eng.keygen(2048, 0)
+    gk = eng.Key.from_keygen(ctx, kd, r_bits=256, window_bits=0)
+    ok = oracle_mod.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
+    rng = np.random.default_rng(0)
+    m1 = rng.integers(I64_MIN, I64_MAX, size=4096, dtype=np.int64, endpoint=True)
+    m2 = rng.integers(I64_MIN, I64_MAX, size=4096, dtype=np.int64, endpoint=True)
+    r1, r2 = eng.seeded_r(0, 1, 4096, 256), eng.seeded_r(0, 2, 4096, 256)
+    print("config 1: encrypt x2, add, decrypt (4096)", flush=True)
+    c1, c2 = gk.encrypt_i64(m1, r1), gk.encrypt_i64(m2, r2)
+    s = gk.add(c1, c2)
+    d = gk.decrypt(s, out_words=3)
+    gk.ctx.sync()
+    o1, o2 = ok.encrypt_i64(m1, r1, threads=8), ok.encrypt_i64(m2, r2, threads=8)
+    assert np.array_equal(host(c1), o1) and np.array_equal(host(c2), o2)
+    os_ = ok.add(o1, o2, threads=8)
+    assert np.array_equal(host(s), os_)
+    assert np.array_equal(host(d), ok.decrypt(os_, out_words=3, threads=8))
+    assert [eng.twos_to_int(row) for row in host(d)] == [int(a) + int(b) for a, b in zip(m1, m2)]
+    gk.close()
