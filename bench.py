@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy kernel (HBM counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the key's default parameters (automatic table window, 2-lane decryption)")
     ap.add_argument("--configs", default="3,4,5",
                     help="batched / served configurations to run after the headline one (comma list, '' for none)")
     return ap.parse_args()
@@ -143,7 +145,8 @@ def main():
     ctx = E.Context(0)
     kd = E.keygen(2048, 0)
     key = E.Key.from_keygen(ctx, kd, r_bits=256, window_bits=0)  # widest table under 1 GiB
-    tune = key.autotune()  # table window, decryption lanes, work-list grid: the fastest on this device
+    # table window, decryption lanes, work-list grid: the fastest on this device
+    tune = None if args.no_autotune else key.autotune()
     rng = np.random.default_rng(0)
     m_h = rng.integers(-(2**63), 2**63 - 1, size=N, dtype=np.int64, endpoint=True)
     r_h = E.seeded_r(0, 11, N, 256)
