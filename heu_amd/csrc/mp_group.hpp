@@ -188,9 +188,35 @@ __device__ __forceinline__ void norm_step(u64 (&acc)[K], u32 lid) {
 //   the low 29 bits of acc_0 move to lane-1's top slot.
 // In group lane 0, acc_0 = 0 mod 2^29, so nothing is lost there; in other
 // lanes the split keeps the value: acc_0 = lo + 2^29 * hi.
-template <int L, int K>
-__device__ __forceinline__ void mont_mul(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
-                                         const GMod<L, K>& md) {
+//
+// Squaring (kSqr, b = the lane's limbs of the LDS operand a): at iteration i,
+// lane l multiplies a_i by its limbs a_{lK+j}, so the full product forms every
+// cross term a_x a_y twice, at iteration x and at iteration y.  Which products
+// an unrolled iteration issues must not depend on the lane, so the choice is a
+// function of (i mod K, j) only:
+//   j == i mod K: a_i * a_j as in the full product (the square a_i^2 in one
+//     lane; cross terms of equal residues, formed at both iterations);
+//   otherwise: 2 a_i * a_j if sq_cross_here<K>(i mod K, j), else nothing.
+// sq_cross_here is a tournament on the K residues (exactly one of (x, y),
+// (y, x) holds), so every other cross term is formed once, doubled.  An
+// iteration then issues (K+1)/2 product multiply-adds instead of K.  A slot
+// gains < 2^59 + 2^58 (+ 2^36) per iteration, so the carry step comes every 21
+// iterations: 21 * 3 * 2^58 < 2^64 - 2^58.  Every position's products are all
+// added by the iteration that consumes it, so the quotients, and the result,
+// equal the full product's.
+constexpr int kNormEverySq = 21;
+
+template <int K>
+constexpr bool sq_cross_here(int x, int y) {
+  const int d = (y - x + K) % K;
+  if constexpr (K & 1) return d >= 1 && d <= (K - 1) / 2;
+  return (d >= 1 && d < K / 2) || (d == K / 2 && x < K / 2);
+}
+
+template <int L, int K, bool kSqr>
+__device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
+                                          const GMod<L, K>& md) {
+  constexpr int kNorm = kSqr ? kNormEverySq : kNormEvery;
 #pragma unroll
   for (int t = 0; t < K; ++t) acc[t] = 0;
   const u32 lid = grp_lane<L>();
@@ -202,8 +228,19 @@ __device__ __forceinline__ void mont_mul(u64 (&acc)[K], const u32* __restrict__ 
     static_for<K>([&](auto S) {
       constexpr int s = decltype(S)::value;
       const u32 ai = ap[s];
+      if constexpr (kSqr) {
+        const u32 ai2 = ai << 1;
+        static_for<K>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          if constexpr (j == s)
+            acc[(j + s) % K] = mad64(ai, b[j], acc[(j + s) % K]);
+          else if constexpr (sq_cross_here<K>(s, j))
+            acc[(j + s) % K] = mad64(ai2, b[j], acc[(j + s) % K]);
+        });
+      } else {
 #pragma unroll
-      for (int j = 0; j < K; ++j) acc[(j + s) % K] = mad64(ai, b[j], acc[(j + s) % K]);
+        for (int j = 0; j < K; ++j) acc[(j + s) % K] = mad64(ai, b[j], acc[(j + s) % K]);
+      }
       u32 q = ((u32)acc[s % K] * md.mprime) & kLimbMask;
       q = grp_bcast0<L>(q);
 #pragma unroll
@@ -211,9 +248,22 @@ __device__ __forceinline__ void mont_mul(u64 (&acc)[K], const u32* __restrict__ 
       const u64 out0 = acc[s % K];
       acc[(s + 1) % K] += out0 >> kLimbBits;
       acc[s % K] = (u64)grp_shl1<L>((u32)out0 & kLimbMask, lid);
-      if constexpr ((s + 1) % kNormEvery == 0 || s == K - 1) norm_step<L, K, (s + 1) % K>(acc, lid);
+      if constexpr ((s + 1) % kNorm == 0 || s == K - 1) norm_step<L, K, (s + 1) % K>(acc, lid);
     });
   }
+}
+
+template <int L, int K>
+__device__ __forceinline__ void mont_mul(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
+                                         const GMod<L, K>& md) {
+  mont_core<L, K, false>(acc, a_lds, b, md);
+}
+
+// acc = a * a * R^-1 (mod m), lazy; b holds this lane's limbs of a_lds.
+template <int L, int K>
+__device__ __forceinline__ void mont_sqr(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
+                                         const GMod<L, K>& md) {
+  mont_core<L, K, true>(acc, a_lds, b, md);
 }
 
 // Lazy accumulators -> limbs each < 2^29 + 65 (value unchanged; value < 2^(29*NL)).

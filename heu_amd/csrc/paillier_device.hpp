@@ -71,7 +71,11 @@ struct Grp {
   __device__ __forceinline__ void square() {
     u32 b[K];
     get_lds<L, K>(b, A, lid);
-    mul(b);
+    u64 acc[K];
+    mont_sqr<L, K>(acc, A, b, md);
+    u32 x[K];
+    seminorm<L, K>(acc, x);
+    put_lds<L, K>(A, x, lid);
   }
   __device__ __forceinline__ void mul_B() {
     u32 b[K];
