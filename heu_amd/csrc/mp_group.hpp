@@ -12,9 +12,16 @@
 // as an add, so the loop is built to spend as few non-multiply instructions as
 // possible per product: every 29x29-bit product is one v_mad_u64_u32 into its
 // accumulator with no carry chain.  An accumulator gains < 2^59.01 per outer
-// iteration, so a one-step carry normalisation every 31 iterations keeps all of
-// them below 2^64.  With R = 2^(29*NL) >= 4m, inputs below 2m give outputs
-// below 2m, so chained multiplications never need the conditional
+// iteration, and lives in one lane for at most K iterations: it enters as the
+// lane's top slot holding < 2^29, and leaves as slot 0, whose bits above 29 go
+// to slot 1 of the same lane and whose low 29 bits go to the lane below.  So
+// for K <= 31 no carry step is needed at all.  K = 36 multiplies take a
+// one-step carry over all slots every 31 iterations; K = 36 squarings (whose
+// bound is 27 iterations, below) carry one slot per iteration instead, the one
+// in the middle of the lane, which hands its bits above 29 to the slot above it
+// and so cuts every slot's life into runs of <= K/2 + 1 iterations between
+// carries.  With R = 2^(29*NL) >= 4m, inputs below 2m give
+// outputs below 2m, so chained multiplications never need the conditional
 // subtraction; only results leaving the engine are canonicalised.
 //
 // Cross-lane traffic per outer iteration: one DPP broadcast of the Montgomery
@@ -35,7 +42,7 @@ typedef int64_t i64;
 
 constexpr int kLimbBits = 29;
 constexpr u32 kLimbMask = (1u << kLimbBits) - 1;
-constexpr int kNormEvery = 31;  // iterations between carry steps (see bound above)
+constexpr int kNormEvery = 31;  // longest run of multiply iterations without a carry (see bound above)
 
 // ---------------------------------------------------------------------------
 // DPP controls (GFX9 encoding) and group primitives
@@ -199,12 +206,16 @@ __device__ __forceinline__ void norm_step(u64 (&acc)[K], u32 lid) {
 //   otherwise: 2 a_i * a_j if sq_cross_here<K>(i mod K, j), else nothing.
 // sq_cross_here is a tournament on the K residues (exactly one of (x, y),
 // (y, x) holds), so every other cross term is formed once, doubled.  An
-// iteration then issues (K+1)/2 product multiply-adds instead of K.  A slot
-// gains < 2^59 + 2^58 (+ 2^36) per iteration, so the carry step comes every 21
-// iterations: 21 * 3 * 2^58 < 2^64 - 2^58.  Every position's products are all
-// added by the iteration that consumes it, so the quotients, and the result,
-// equal the full product's.
-constexpr int kNormEverySq = 21;
+// iteration then issues (K+1)/2 product multiply-adds instead of K.  Every
+// position's products are all added by the iteration that consumes it, so the
+// quotients, and the result, equal the full product's.
+// Bound: in units of 2^58, an accumulator gains < 1 per iteration from q*m and
+// 2 (doubled), 1 (j == i mod K) or 0 from the product.  Enumerating the
+// schedule, any G consecutive iterations of one slot in one lane total at most
+// G + P(G) units with G + P(G) <= 63 for G <= 27 (P(G) = the largest sum of
+// product weights over such a window), so slots stay below 2^64 in runs of up
+// to 27 iterations between carries, and K <= 27 needs none.
+constexpr int kNormEverySq = 27;
 
 template <int K>
 constexpr bool sq_cross_here(int x, int y) {
@@ -217,6 +228,13 @@ template <int L, int K, bool kSqr>
 __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
                                           const GMod<L, K>& md) {
   constexpr int kNorm = kSqr ? kNormEverySq : kNormEvery;
+  // Carries when K > kNorm.  Squaring: the middle slot each iteration (runs of
+  // K-1-kMid and kMid+1 iterations).  Multiplication: a carry step over all
+  // slots every kNorm iterations and at the end of each block (the per-
+  // iteration form makes the compiler interleave iterations until the K = 36
+  // multiply spills registers).
+  constexpr int kMid = K / 2;
+  static_assert(K <= kNorm || (K - 1 - kMid <= kNorm && kMid + 1 <= kNorm), "carry runs too long");
 #pragma unroll
   for (int t = 0; t < K; ++t) acc[t] = 0;
   const u32 lid = grp_lane<L>();
@@ -248,7 +266,14 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
       const u64 out0 = acc[s % K];
       acc[(s + 1) % K] += out0 >> kLimbBits;
       acc[s % K] = (u64)grp_shl1<L>((u32)out0 & kLimbMask, lid);
-      if constexpr ((s + 1) % kNorm == 0 || s == K - 1) norm_step<L, K, (s + 1) % K>(acc, lid);
+      if constexpr (K > kNorm && kSqr) {
+        // the slot that is logical kMid in the next iteration
+        u64& x = acc[(kMid + s + 1) % K];
+        acc[(kMid + 1 + s + 1) % K] += x >> kLimbBits;
+        x &= kLimbMask;
+      } else if constexpr (K > kNorm && ((s + 1) % kNorm == 0 || s == K - 1)) {
+        norm_step<L, K, (s + 1) % K>(acc, lid);
+      }
     });
   }
 }
