@@ -105,6 +105,21 @@ __device__ __forceinline__ u32 grp_shr1(u32 x, u32 lid) {
   return v;
 }
 
+// Lane i receives lane i+1's value and the top lane receives group lane 0's.
+// For groups within a quad (L <= 4) this is one rotation inside the group;
+// Montgomery's limb shift uses it where group lane 0 holds 0, which then
+// arrives in the top lane with no select.  Wider groups shift (top lane <- 0).
+template <int L>
+__device__ __forceinline__ u32 grp_shl1_z(u32 x, u32 lid) {
+  if constexpr (L == 2) {
+    return __builtin_amdgcn_update_dpp(x, x, quad_perm(1, 0, 3, 2), 0xf, 0xf, false);
+  } else if constexpr (L == 4) {
+    return __builtin_amdgcn_update_dpp(x, x, quad_perm(1, 2, 3, 0), 0xf, 0xf, false);
+  } else {
+    return grp_shl1<L>(x, lid);
+  }
+}
+
 template <int L>
 __device__ __forceinline__ u64 grp_shr1_64(u64 x, u32 lid) {
   u32 lo = grp_shr1<L>((u32)x, lid), hi = grp_shr1<L>((u32)(x >> 32), lid);
@@ -193,8 +208,9 @@ __device__ __forceinline__ void norm_step(u64 (&acc)[K], u32 lid) {
 //   acc += a_i * b;  q = acc_0 * m' mod 2^29 (group lane 0, broadcast);
 //   acc += q * m;    acc_1 += acc_0 >> 29 in every lane;
 //   the low 29 bits of acc_0 move to lane-1's top slot.
-// In group lane 0, acc_0 = 0 mod 2^29, so nothing is lost there; in other
-// lanes the split keeps the value: acc_0 = lo + 2^29 * hi.
+// In group lane 0, acc_0 = 0 mod 2^29, so nothing is lost there (and the zero
+// it sends is what the top lane receives); in other lanes the split keeps the
+// value: acc_0 = lo + 2^29 * hi.
 //
 // Squaring (kSqr, b = the lane's limbs of the LDS operand a): at iteration i,
 // lane l multiplies a_i by its limbs a_{lK+j}, so the full product forms every
@@ -265,7 +281,7 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
       for (int j = 0; j < K; ++j) acc[(j + s) % K] = mad64(q, m[j], acc[(j + s) % K]);
       const u64 out0 = acc[s % K];
       acc[(s + 1) % K] += out0 >> kLimbBits;
-      acc[s % K] = (u64)grp_shl1<L>((u32)out0 & kLimbMask, lid);
+      acc[s % K] = (u64)grp_shl1_z<L>((u32)out0 & kLimbMask, lid);  // group lane 0 sends 0
       if constexpr (K > kNorm && kSqr) {
         // the slot that is logical kMid in the next iteration
         u64& x = acc[(kMid + s + 1) % K];
