@@ -57,6 +57,14 @@ def r_limbs(config, seed, stream, count) -> np.ndarray:
     return E.seeded_r((config << 32) | seed, stream, count, R_BITS)
 
 
+def chain_expected(m: np.ndarray, m2: np.ndarray, k: np.ndarray) -> np.ndarray:
+    """(m + m2) * k per element as 128-bit two's complement, 4 uint32 words each:
+    what decrypting (Enc(m) (+) Enc(m2)) (x) k must give, from Python integers."""
+    vals = ((int(a) + int(b)) * int(c) for a, b, c in zip(m.tolist(), m2.tolist(), k.tolist()))
+    raw = b"".join(v.to_bytes(16, "little", signed=True) for v in vals)
+    return np.frombuffer(raw, dtype=np.uint32).reshape(len(m), 4)
+
+
 def _dev(a, dtype=None):
     t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
     return t if dtype is None else t.to(dtype)

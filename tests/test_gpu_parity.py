@@ -255,3 +255,29 @@ def test_config1_workload(eng, oracle_mod, ctx):
     assert np.array_equal(host(d), ok.decrypt(os_, out_words=3, threads=8))
     assert [eng.twos_to_int(row) for row in host(d)] == [int(a) + int(b) for a, b in zip(m1, m2)]
     gk.close()
+
+
+def test_chain_full_size(eng, ctx, k2048):
+    # the headline workload at its full size (1,048,576 elements, one op per
+    # launch): Dec((Enc(m) (+) Enc(m')) (x) k) = (m + m') k for every element,
+    # a closed form independent of the size; plus a 4,096-element slice of the
+    # ciphertexts bit-exact with the oracle (the rest is covered by the closed
+    # form: a wrong ciphertext does not decrypt to it)
+    from heu_amd.configs import chain_expected
+
+    kd, gk, ok = k2048
+    n = 1 << 20
+    rng = np.random.default_rng(20)
+    m = rng.integers(I64_MIN, I64_MAX, size=n, dtype=np.int64, endpoint=True)
+    m[: len(EDGE_I64)] = EDGE_I64
+    k = rng.integers(-(2**31), 2**31 - 1, size=n, dtype=np.int64, endpoint=True)
+    k[:4] = [0, 1, -1, -(2**31)]
+    r = eng.seeded_r(20, 11, n, 256)
+    print("full-size chain: encrypt, ct+ct, ct x int32, decrypt (1,048,576)", flush=True)
+    c1 = gk.encrypt_i64(m, r)
+    c2 = gk.encrypt_i64(np.ascontiguousarray(m[::-1]), r)
+    p = gk.mul_i64(gk.add(c1, c2), k)
+    d = gk.decrypt(p, out_words=4)
+    gk.ctx.sync()
+    assert np.array_equal(host(d), chain_expected(m, m[::-1], k))
+    assert np.array_equal(host(c1[:4096]), ok.encrypt_i64(m[:4096], r[:4096], threads=8))
