@@ -390,31 +390,38 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_add_pt_i64(P
 }
 
 // ---------------------------------------------------------------------------
-// Powers with 64-bit exponents, fixed 4-bit windows in Montgomery space.
-// T (16 entries of NL words, global scratch) receives c^d R for d < 16.
-// A holds c R on entry and c^mag R on return.  Leading zero windows are
-// skipped per group.
+// Powers with 64-bit exponents, fixed 3-bit windows in Montgomery space.
+// T (global scratch, room for 2^kPowWin entries of NL words) receives c^d R
+// for d < 8.  A holds c R on entry and c^mag R on return.  Leading zero
+// windows are skipped per group.  Three bits beat four for the common 32-bit
+// magnitudes (6 + 10 products and 30 squarings against 14 + 7 and 28, 8% less
+// work at a product's cost of 1.26 squarings) and tie at 64 bits.
 // ---------------------------------------------------------------------------
+constexpr int kPowWinI64 = 3;
+static_assert(kPowWinI64 <= kPowWin, "table room");
+
 template <int L, int K, int W>
 __device__ __forceinline__ void pow_u64(Grp<L, K, W>& g, const ModArgs& mod, u64 mag, u32* __restrict__ T) {
   constexpr int NL = L * K;
+  constexpr int kW = kPowWinI64;
+  constexpr u32 kMask = (1u << kW) - 1u;
   u32 x[K];
   load_r29<L, K>(x, mod.one, g.lid);
   store_r29<L, K>(T, x, g.lid);
   get_lds<L, K>(x, g.A, g.lid);
   store_r29<L, K>(T + NL, x, g.lid);
   g.set(g.B, x);
-  for (int d = 2; d < (1 << kPowWin); ++d) {
+  for (int d = 2; d < (1 << kW); ++d) {
     g.mul_B();  // A = c^d R
     g.save(T + d * NL, g.A);
   }
-  int top = 64 / kPowWin - 1;
-  while (top > 0 && ((mag >> (kPowWin * top)) == 0)) --top;
-  g.set_global(g.A, T + ((u32)(mag >> (kPowWin * top)) & ((1u << kPowWin) - 1u)) * NL);
+  int top = (64 + kW - 1) / kW - 1;  // the top window holds bit 63
+  while (top > 0 && ((mag >> (kW * top)) == 0)) --top;
+  g.set_global(g.A, T + ((u32)(mag >> (kW * top)) & kMask) * NL);
   for (int w = top - 1; w >= 0; --w) {
 #pragma unroll 1
-    for (int s = 0; s < kPowWin; ++s) g.square();
-    g.mul_global(T + ((u32)(mag >> (kPowWin * w)) & ((1u << kPowWin) - 1u)) * NL);
+    for (int s = 0; s < kW; ++s) g.square();
+    g.mul_global(T + ((u32)(mag >> (kW * w)) & kMask) * NL);
   }
 }
 

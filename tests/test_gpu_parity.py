@@ -104,7 +104,7 @@ def test_mul_scalar(eng, which, request):
     a = ok.encrypt_i64(m, r)
     rng = np.random.default_rng(9)
     k = rng.integers(-(2**31), 2**31 - 1, size=48, dtype=np.int64)
-    k[:8] = [0, 1, -1, 2, -2, 3, I64_MAX, I64_MIN + 1]
+    k[:11] = [0, 1, -1, 2, -2, 3, I64_MAX, I64_MIN + 1, I64_MIN, 7, -8]  # I64_MIN: |k| = 2^63, the top window
     print(f"[{which}] mul_i64 n=48", flush=True)
     out = gk.mul_i64(a, k)
     gk.ctx.sync()
