@@ -6,9 +6,12 @@
 // negative-scalar powers, upper reduction levels, segment finishes) becomes a
 // list of steps in task order.  The planner runs twice: a sizing pass that only
 // counts workspace bytes, then, with the workspace allocated, the pass that
-// produces device pointers, uploads the work lists and records the steps.  A
-// run replays the steps on the context's stream, or launches the hipGraph they
-// were captured into.
+// produces device pointers, uploads the work lists and records the steps.
+// Within a stage, each work-list launch and the steps that read its results
+// form a lane; lanes of one stage run on streams of their own (forked from the
+// context's stream and joined back), so one shape's tail of long units
+// overlaps another shape's work.  A run replays the stages, or launches the
+// hipGraph they were captured into (the fork and join become graph edges).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,9 +32,11 @@ using namespace heu_engine;
 struct heu_batch {
   heu_ctx* ctx = nullptr;
   DevBuf ws;  // workspace: work lists, chunk lists, partials, window tables
-  std::vector<std::function<void(hipStream_t)>> steps;
+  std::vector<PlanStage> stages;
   uint32_t flags = 0;
   uint64_t launches = 0, units = 0;
+  std::vector<hipStream_t> side;   // lanes 1.. of a stage (lane 0 runs on the main stream)
+  std::vector<hipEvent_t> events;  // per multi-lane stage: one fork, one join per side lane
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -47,8 +52,10 @@ struct Builder {
   char* base = nullptr;
   size_t used = 0;
   std::vector<std::pair<size_t, std::vector<char>>> metas;  // host images to upload
-  std::vector<Step> follow;                                 // this stage's steps after its work lists
+  std::vector<Step>* cur = nullptr;                         // the lane a task's steps join
+  std::vector<Step> follow;                                 // steps after every lane of the stage
   std::vector<Step> steps;                                  // every stage, in order
+  std::vector<PlanStage> stages;
   uint64_t launches = 0;
 
   template <class T>
@@ -69,7 +76,7 @@ struct Builder {
   }
   void then(Step s, uint64_t nlaunch = 1) {
     launches += nlaunch;
-    if (base) follow.push_back(std::move(s));
+    if (base) (cur ? *cur : follow).push_back(std::move(s));
   }
 };
 
@@ -80,6 +87,7 @@ struct WorkList {
   int key_bits = 0;
   bool dec = false;
   std::vector<std::pair<double, WorkItem>> items;
+  std::vector<Step> follow;  // steps reading this launch's results, in task order
 };
 
 // Montgomery products per unit (one element or chunk per group), for ordering.
@@ -296,9 +304,12 @@ int plan_stage(Builder& b, heu_ctx* ctx, const heu_task* tasks, size_t ntasks, u
         it.b = t.b;
         it.out = t.out;
         it.neg = neg;
-        list_for(key, false).items.emplace_back(unit_cost(it.op), it);
+        WorkList& wl = list_for(key, false);
+        wl.items.emplace_back(unit_cost(it.op), it);
         uint32_t* out = static_cast<uint32_t*>(t.out);
+        b.cur = &wl.follow;
         plan_invert(b, key, out, neg, n, out);
+        b.cur = nullptr;
         break;
       }
       case HEU_OP_DECRYPT: {
@@ -318,14 +329,20 @@ int plan_stage(Builder& b, heu_ctx* ctx, const heu_task* tasks, size_t ntasks, u
         const heu_hip::CombArgs C = key->comb();
         uint32_t* out = static_cast<uint32_t*>(t.out);
         const int ow = (int)t.aux;
+        b.cur = &list_for(key, true).follow;
         b.then([=](hipStream_t st) { o->combine(st, C, halves, n, out, ow, nullptr); });
+        b.cur = nullptr;
         break;
       }
       case HEU_OP_SUM:
-      case HEU_OP_DOT_I64:
+      case HEU_OP_DOT_I64: {
         if (t.segments == 0) break;
-        plan_segmented(b, key, t, list_for(key, false));
+        WorkList& wl = list_for(key, false);
+        b.cur = &wl.follow;
+        plan_segmented(b, key, t, wl);
+        b.cur = nullptr;
         break;
+      }
       default:
         break;
     }
@@ -333,7 +350,7 @@ int plan_stage(Builder& b, heu_ctx* ctx, const heu_task* tasks, size_t ntasks, u
   // one persistent launch per shape, expensive units first
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-  std::vector<Step> launches;
+  std::vector<std::pair<double, std::vector<Step>>> lanes;  // (estimated work, steps)
   uint64_t units = 0;
   for (auto& kv : lists) {
     WorkList& wl = kv.second;
@@ -345,10 +362,16 @@ int plan_stage(Builder& b, heu_ctx* ctx, const heu_task* tasks, size_t ntasks, u
     const long gpu = wl.shape->groups_per_unit;
     std::vector<WorkItem> items;
     long u0 = 0;
+    double work = 0;
+    // unit costs count products mod n^2 (mod p^2 for decryption, a quarter of
+    // the work); a 3072-bit product is 2.25 times a 2048-bit one
+    const double scale = (wl.key_bits == 3072 ? 2.25 : 1.0) * (wl.dec ? 0.25 : 1.0);
     for (auto& ci : wl.items) {
       WorkItem it = ci.second;
       it.unit0 = u0;
-      u0 += (it.count + gpu - 1) / gpu;
+      const long nu = (it.count + gpu - 1) / gpu;
+      u0 += nu;
+      work += ci.first * (double)nu * (double)wl.shape->groups_per_unit * scale;
       items.push_back(it);
     }
     if (u0 >= (long)UINT32_MAX) return fail(HEU_EINVAL, "batch too large for one work list");
@@ -362,14 +385,29 @@ int plan_stage(Builder& b, heu_ctx* ctx, const heu_task* tasks, size_t ntasks, u
     const heu_hip::WorklistFn fn = wl.shape->launch;
     const int nitems = (int)items.size();
     b.launches += 1;
-    if (b.base)
-      launches.push_back([=](hipStream_t st) { fn(st, d_items, nitems, total, ctr, slot_tab, grid); });
+    if (b.base) {
+      std::vector<Step> lane{[=](hipStream_t st) { fn(st, d_items, nitems, total, ctr, slot_tab, grid); }};
+      lane.insert(lane.end(), wl.follow.begin(), wl.follow.end());
+      lanes.emplace_back(work, std::move(lane));
+    }
     units += total;
   }
   if (b.base) {
-    b.steps.insert(b.steps.end(), launches.begin(), launches.end());
+    // the most work first: its launch is issued first and takes the chip, and
+    // the lighter lanes fill in as its workgroups drain
+    std::stable_sort(lanes.begin(), lanes.end(),
+                     [](const std::pair<double, std::vector<Step>>& x, const std::pair<double, std::vector<Step>>& y) {
+                       return x.first > y.first;
+                     });
+    PlanStage stage;
+    for (auto& l : lanes) {
+      b.steps.insert(b.steps.end(), l.second.begin(), l.second.end());
+      stage.lanes.push_back(std::move(l.second));
+    }
     b.steps.insert(b.steps.end(), b.follow.begin(), b.follow.end());
+    stage.after = std::move(b.follow);
     b.follow.clear();
+    if (!stage.lanes.empty() || !stage.after.empty()) b.stages.push_back(std::move(stage));
   }
   *units_out += units;
   return HEU_OK;
@@ -415,10 +453,48 @@ int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, P
   out->units = units;
   out->metas = std::move(b.metas);
   out->steps = std::move(b.steps);
+  out->stages = std::move(b.stages);
   return HEU_OK;
 }
 
 }  // namespace heu_engine
+
+namespace {
+
+void release_streams(heu_batch* b) {
+  for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  b->side.clear();
+  b->events.clear();
+}
+
+// Enqueues one run on `main`: stage by stage; a stage with several lanes forks
+// them onto the side streams after an event on `main`, and `main` waits for
+// each side lane's event before the stage's `after` steps and the next stage.
+hipError_t enqueue(heu_batch* b, hipStream_t main) {
+  size_t ev = 0;
+  hipError_t e;
+  for (const PlanStage& st : b->stages) {
+    const size_t nl = st.lanes.size();
+    if (nl > 1) {
+      hipEvent_t fork = b->events[ev++];
+      if ((e = hipEventRecord(fork, main)) != hipSuccess) return e;
+      for (size_t i = 1; i < nl; ++i)
+        if ((e = hipStreamWaitEvent(b->side[i - 1], fork, 0)) != hipSuccess) return e;
+    }
+    for (size_t i = 0; i < nl; ++i)
+      for (const PlanStep& s : st.lanes[i]) s(i ? b->side[i - 1] : main);
+    for (size_t i = 1; i < nl; ++i) {
+      hipEvent_t join = b->events[ev++];
+      if ((e = hipEventRecord(join, b->side[i - 1])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(main, join, 0)) != hipSuccess) return e;
+    }
+    for (const PlanStep& s : st.after) s(main);
+  }
+  return hipSuccess;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -439,9 +515,29 @@ int heu_batch_create(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, uint32_
     HIP_TRY(hipMemcpyAsync(batch->ws.as<char>() + m.first, m.second.data(), m.second.size(), hipMemcpyHostToDevice,
                            ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));  // the host images die with `p`
-  batch->steps = std::move(p.steps);
+  batch->stages = std::move(p.stages);
   batch->launches = p.launches;
   batch->units = p.units;
+  size_t lanes = 1, nev = 0;
+  for (const PlanStage& st : batch->stages) {
+    lanes = std::max(lanes, st.lanes.size());
+    if (st.lanes.size() > 1) nev += st.lanes.size();
+  }
+  rc = HEU_OK;
+  for (size_t i = 1; i < lanes && !rc; ++i) {
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) rc = fail(HEU_EDEVICE, "hipStreamCreate failed");
+    else batch->side.push_back(s);
+  }
+  for (size_t i = 0; i < nev && !rc; ++i) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) rc = fail(HEU_EDEVICE, "hipEventCreate failed");
+    else batch->events.push_back(e);
+  }
+  if (rc) {
+    release_streams(batch.get());
+    return rc;
+  }
   ctx->refs += 1;  // under ctx->mu: the batch holds the context
   *out = batch.release();
   return HEU_OK;
@@ -454,7 +550,7 @@ int heu_batch_run(heu_batch* batch) {
   HIP_TRY(hipSetDevice(ctx->device));
   (void)hipGetLastError();  // an error left by other code is not this run's
   if (!(batch->flags & HEU_BATCH_GRAPH)) {
-    for (auto& s : batch->steps) s(ctx->stream);
+    HIP_TRY(enqueue(batch, ctx->stream));
     HIP_TRY(hipGetLastError());
     return HEU_OK;
   }
@@ -462,10 +558,12 @@ int heu_batch_run(heu_batch* batch) {
     // capture on a private stream (the context's stream may be the null stream)
     if (!batch->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&batch->cap_stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamBeginCapture(batch->cap_stream, hipStreamCaptureModeThreadLocal));
-    for (auto& s : batch->steps) s(batch->cap_stream);
+    const hipError_t qe = enqueue(batch, batch->cap_stream);
     hipGraph_t g = nullptr;
     const hipError_t le = hipGetLastError();
-    HIP_TRY(hipStreamEndCapture(batch->cap_stream, &g));
+    const hipError_t ce = hipStreamEndCapture(batch->cap_stream, &g);
+    HIP_TRY(qe);
+    HIP_TRY(ce);
     HIP_TRY(le);
     batch->graph = g;
     HIP_TRY(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0));
@@ -484,6 +582,7 @@ int heu_batch_destroy(heu_batch* batch) {
     if (batch->exec) (void)hipGraphExecDestroy(batch->exec);
     if (batch->graph) (void)hipGraphDestroy(batch->graph);
     if (batch->cap_stream) (void)hipStreamDestroy(batch->cap_stream);
+    release_streams(batch);
     delete batch;
   }
   ctx_release(ctx);

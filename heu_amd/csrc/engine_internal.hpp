@@ -66,12 +66,22 @@ std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, cons
 // A planned batch (batch.cpp): with base == nullptr only `bytes` is computed
 // (the sizing pass); otherwise every device pointer lies in [base, base +
 // bytes), `metas` are host images to copy to base + offset before the first
-// run, and `steps` enqueue one run on a stream.
+// run, and `steps` enqueue one run on a stream.  `stages` hold the same steps
+// with their dependencies: stages run in order; within a stage each lane (one
+// work-list launch and the steps that read its results) is independent of the
+// other lanes, so lanes may run on streams of their own, and `after` runs once
+// every lane is done.
+typedef std::function<void(hipStream_t)> PlanStep;
+struct PlanStage {
+  std::vector<std::vector<PlanStep>> lanes;
+  std::vector<PlanStep> after;
+};
 struct PlanOut {
   size_t bytes = 0;
   uint64_t launches = 0, units = 0;
   std::vector<std::pair<size_t, std::vector<char>>> metas;
-  std::vector<std::function<void(hipStream_t)>> steps;
+  std::vector<PlanStep> steps;
+  std::vector<PlanStage> stages;
 };
 // Context lifetime: objects created on a context hold a reference; the
 // context is freed by heu_ctx_destroy or by the last release after it.
