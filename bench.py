@@ -52,8 +52,19 @@ ALGO_MACS = {
     "mulmod": M(128),                       # ct + ct
     "pow_i64": 44 * M(128),                 # ct x int32 (algorithmic count: 44 products)
     "dec_half": 2 * (1023 + 171 + 16) * M(64),  # both CRT exponentiations per element
+    "reduce_sum": M(128),                   # one product per summed ciphertext
+}
+# Algorithmic HBM bytes per element (inputs read + outputs written; table and
+# window-table reads are reuse traffic, not algorithmic).
+ALGO_BYTES = {
+    "encrypt_i64": 8 + 32 + 512,
+    "mulmod": 3 * 512,
+    "pow_i64": 516 + 512,
+    "dec_half": 512 + 16,
+    "reduce_sum": 512,                      # per input ciphertext
 }
 VALU_MAC_PEAK_T = 64 * 256 * 2.4e9 / 1e12   # 39.32 T MAC/s
+HBM_PEAK_GBS = 8000.0
 
 
 def parse():
@@ -203,19 +214,48 @@ def main():
         if launches:
             kern[name] = (total, launches)
     ctx.profile(False)
+    # streaming sum: the resident 1M-ciphertext vector as a column-major 4096 x 256
+    # HeMatrix, summed along axis 0 (256 column products), twice
+    offs = np.append(np.arange(0, N, 4096, dtype=np.int64), np.int64(N))
+    key.sum(ct_res, offs)
+    ctx.sync()
+    ctx.profile(True)
+    for _ in range(2):
+        key.sum(ct_res, offs)
+    ctx.sync()
+    sum_ms, _ = ctx.profile_read("reduce_sum")
+    ctx.profile(False)
+    kernels = {}
+    traffic_all = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic_all = json.load(open(tpath))
+        except Exception:
+            traffic_all = {}
+    for name, (total, launches) in list(kern.items()) + ([("reduce_sum", (sum_ms, 0))] if sum_ms > 0 else []):
+        elems = 2 * N if name == "reduce_sum" else N * args.steps
+        secs = total / 1e3
+        row = {
+            "ms_total": total,
+            "elements": elems,
+            "t_mac_per_s": ALGO_MACS[name] * elems / secs / 1e12,
+            "valu_frac": ALGO_MACS[name] * elems / secs / 1e12 / VALU_MAC_PEAK_T,
+            "algorithmic_gb_per_s": ALGO_BYTES[name] * elems / secs / 1e9,
+        }
+        if launches:
+            row["avg_launch_ms"] = total / launches
+            row["elements_per_launch"] = elems / launches
+            if name in traffic_all:  # HBM bytes per 131,072-element launch, from the counter profile
+                row["hbm_gb_per_s"] = traffic_all[name] * (elems / launches / 131072) / (total / launches / 1e3) / 1e9
+        kernels[name] = row
     dom = max(kern, key=lambda n: kern[n][0])
     dom_ms, dom_launches = kern[dom]
     elems_per_launch = (N * args.steps) / dom_launches
     macs_per_launch = ALGO_MACS[dom] * elems_per_launch
     avg_s = dom_ms / dom_launches / 1e3
     achieved = macs_per_launch / avg_s / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get(dom)
-        except Exception:
-            traffic = None
+    traffic = traffic_all.get(dom)
 
     ms_per_step = elapsed / args.steps * 1e3
     value = N / (elapsed / args.steps)
@@ -268,6 +308,7 @@ def main():
             "algorithmic_macs_per_element": ALGO_MACS[dom],
         },
         "kernel_ms": {n: kern[n][0] for n in kern},
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -171,14 +171,14 @@ void plan_segmented(Builder& b, heu_key* key, const heu_task& t, WorkList& wl) {
     uint32_t *pos = nullptr, *neg = nullptr;
   };
   std::vector<Level> lv(1);
-  lv[0].ch = make_chunks(start, len, dot ? 8 : 32, &lv[0].first, &lv[0].nch);
+  lv[0].ch = make_chunks(start, len, dot ? kReduceChunkDot0 : kReduceChunk0, &lv[0].first, &lv[0].nch);
   for (;;) {
     const Level& prev = lv.back();
     bool done = true;
     for (size_t s = 0; s < segs; ++s) done &= prev.nch[s] == 1;
     if (done) break;
     Level n;
-    n.ch = make_chunks(prev.first, prev.nch, 32, &n.first, &n.nch);
+    n.ch = make_chunks(prev.first, prev.nch, kReduceChunkUpper, &n.first, &n.nch);
     lv.push_back(std::move(n));
   }
   for (auto& l : lv) {

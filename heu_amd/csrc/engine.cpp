@@ -374,7 +374,7 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
     len[s] = offsets[s + 1] - offsets[s];
     if (len[s] < 0) return fail(HEU_EINVAL, "offsets must be non-decreasing");
   }
-  const long chunk0 = dot ? 8 : 32, chunkU = 32;
+  const long chunk0 = dot ? kReduceChunkDot0 : kReduceChunk0, chunkU = kReduceChunkUpper;
   hipStream_t st = ctx->stream;
   // Plan all levels first (host), then carve scratch once.
   struct Level {

@@ -59,7 +59,13 @@ constexpr int kInvChunk = 32;
 // chunks of kInvChunk, up to one value).
 std::vector<long> inv_levels(long count);
 
-// Chunking of segments for a reduction level.
+// Chunking of segments for a reduction level.  Level 0 reads the inputs in
+// chunks of kReduceChunk0 (kReduceChunkDot0 for dot terms, each a power), which
+// fills the chip for large inputs.  Upper levels see few partials, so a level
+// takes as long as its chunk's sequential products while most of the chip
+// idles: chunks of kReduceChunkUpper keep that chain short (128 partials per
+// segment: 4 + 4 + 4 + 2 products instead of 32 + 4).
+constexpr long kReduceChunk0 = 32, kReduceChunkDot0 = 8, kReduceChunkUpper = 4;
 std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, const std::vector<long>& seg_len, long chunk,
                                         std::vector<long>* seg_first_chunk, std::vector<long>* seg_nchunks);
 
