@@ -82,8 +82,8 @@ def build_spi() -> str:
     binary; both host-only, so g++ builds them."""
     include = os.path.join(os.path.dirname(HERE), "include")
     spi = os.path.join(HERE, "spi")
-    common = ["g++", "-std=c++20", "-O2", "-fPIC", "-Wall", f"-I{include}", f"-I{spi}", f"-I{ROCM}/include",
-              "-D__HIP_PLATFORM_AMD__"]
+    common = ["g++", "-std=c++20", "-O2", "-fPIC", "-Wall", "-pthread", f"-I{include}", f"-I{spi}",
+              f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
     cmd = [*common, "-shared", "-o", SPI_LIB + ".tmp", os.path.join(spi, "paillier_hip.cc"), f"-L{HERE}",
            "-lheu_paillier", f"-L{ROCM}/lib", "-lamdhip64", "-lcrypto", "-Wl,-rpath,$ORIGIN",
            f"-Wl,-rpath,{ROCM}/lib"]

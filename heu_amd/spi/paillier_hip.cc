@@ -3,8 +3,11 @@
 //
 // Every batch call packs its operands into flat limb arrays, stages them in a
 // device arena on the engine context's stream, runs one engine entry point and
-// copies the results back.  Calls from several threads serialize on the
-// process-wide runtime lock (one context, one stream).
+// copies the results back.  Calls from several threads share the process-wide
+// runtime (one context, one stream, one device lock).  Elementwise calls go
+// through a combining queue: a thread that finds no other thread running the
+// queue takes every queued call, and runs each group of calls with the same
+// operation, key and parameters as one staged engine call.
 #include "paillier_hip.h"
 
 #include <hip/hip_runtime_api.h>
@@ -16,7 +19,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <condition_variable>
+#include <exception>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <stdexcept>
 
 #include "heu_paillier.h"
@@ -189,15 +196,37 @@ void hip_ck(hipError_t e, const char* what) {
   if (e != hipSuccess) throw_msg(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// An elementwise call waiting in the combining queue: `count` elements, each
+// input array holding in_elem[i] bytes per element, the output out_elem.
+struct Job {
+  int op = 0;
+  heu_key* key = nullptr;
+  uint32_t param = 0;
+  size_t count = 0;
+  int nin = 0;
+  const void* in[2] = {nullptr, nullptr};
+  size_t in_elem[2] = {0, 0};
+  void* out = nullptr;
+  size_t out_elem = 0;
+  bool done = false;
+  std::exception_ptr err;
+};
+
 // One engine context per process (device from HEU_DEVICE, default 0) and a
 // growable device arena for staging batch operands.  Deliberately never
 // destroyed: HIP's own teardown at exit must not race a late hipFree.
 struct Runtime {
   heu_ctx* ctx = nullptr;
   hipStream_t stream = nullptr;
-  std::mutex mu;
+  std::mutex mu;  // the device: arena, stream, engine calls
   char* arena = nullptr;
   size_t cap = 0;
+  // combining queue
+  std::mutex qmu;
+  std::condition_variable qcv;
+  std::vector<Job*> pending;
+  bool running = false, hold = false;
+  SpiStats stats;
 
   static Runtime& get() {
     static Runtime* rt = [] {
@@ -236,6 +265,8 @@ class DeviceCall {
     return h;
   }
   size_t out(size_t bytes) { return slot(bytes); }
+  // a piece of a slot: upload `bytes` from host at byte offset `off`
+  void put(size_t h, size_t off, const void* host, size_t bytes) { pieces_.push_back({host, bytes, h, off}); }
 
   template <class T>
   T* ptr(size_t h) {
@@ -248,11 +279,15 @@ class DeviceCall {
       if (u.bytes)
         hip_ck(hipMemcpyAsync(base_ + offs_[u.h], u.host, u.bytes, hipMemcpyHostToDevice, rt_.stream),
                "hipMemcpyAsync H2D");
+    for (auto& u : pieces_)
+      if (u.bytes)
+        hip_ck(hipMemcpyAsync(base_ + offs_[u.h] + u.off, u.host, u.bytes, hipMemcpyHostToDevice, rt_.stream),
+               "hipMemcpyAsync H2D");
   }
 
-  void download(size_t h, void* host, size_t bytes) {
+  void download(size_t h, void* host, size_t bytes, size_t off = 0) {
     if (bytes)
-      hip_ck(hipMemcpyAsync(host, base_ + offs_[h], bytes, hipMemcpyDeviceToHost, rt_.stream),
+      hip_ck(hipMemcpyAsync(host, base_ + offs_[h] + off, bytes, hipMemcpyDeviceToHost, rt_.stream),
              "hipMemcpyAsync D2H");
   }
 
@@ -264,6 +299,11 @@ class DeviceCall {
     size_t bytes;
     size_t h;
   };
+  struct Piece {
+    const void* host;
+    size_t bytes;
+    size_t h, off;
+  };
   size_t slot(size_t bytes) {
     offs_.push_back(total_);
     total_ += (bytes + 255) & ~size_t(255);
@@ -274,11 +314,141 @@ class DeviceCall {
   std::lock_guard<std::mutex> lock_;
   std::vector<size_t> offs_;
   std::vector<Up> ups_;
+  std::vector<Piece> pieces_;
   size_t total_ = 0;
   char* base_ = nullptr;
 };
 
+// Operations the combining queue runs (elementwise, fixed bytes per element).
+enum JobOp { kJobEncI64, kJobEncZero, kJobRandomize, kJobDecrypt, kJobAdd, kJobSub, kJobAddI64, kJobMulI64, kJobNegate };
+
+int engine_call(const Job& j, void* const* in, size_t n, void* out) {
+  auto u32 = [](void* p) { return static_cast<uint32_t*>(p); };
+  auto i64 = [](void* p) { return static_cast<int64_t*>(p); };
+  switch (j.op) {
+    case kJobEncI64: return heu_encrypt_i64(j.key, i64(in[0]), u32(in[1]), j.param, n, u32(out));
+    case kJobEncZero: return heu_randomize(j.key, nullptr, u32(in[0]), j.param, n, u32(out));
+    case kJobRandomize: return heu_randomize(j.key, u32(in[0]), u32(in[1]), j.param, n, u32(out));
+    case kJobDecrypt: return heu_decrypt(j.key, u32(in[0]), n, u32(out), j.param, nullptr);
+    case kJobAdd: return heu_add(j.key, u32(in[0]), u32(in[1]), n, u32(out));
+    case kJobSub: return heu_sub(j.key, u32(in[0]), u32(in[1]), n, u32(out));
+    case kJobAddI64: return heu_add_i64(j.key, u32(in[0]), i64(in[1]), n, u32(out));
+    case kJobMulI64: return heu_mul_i64(j.key, u32(in[0]), i64(in[1]), n, u32(out));
+    case kJobNegate: return heu_negate(j.key, u32(in[0]), n, u32(out));
+    default: return HEU_EINVAL;
+  }
+}
+
+// Runs a group of jobs with the same op, key and parameter as one staged
+// engine call.  If it fails, the jobs run one by one, so an error reaches only
+// the call whose inputs caused it.
+void run_group(const std::vector<Job*>& g) {
+  const Job& j0 = *g[0];
+  size_t total = 0;
+  for (const Job* j : g) total += j->count;
+  try {
+    DeviceCall call;
+    size_t hin[2] = {0, 0};
+    for (int i = 0; i < j0.nin; ++i) {
+      hin[i] = call.out(total * j0.in_elem[i]);
+      size_t off = 0;
+      for (const Job* j : g) {
+        call.put(hin[i], off, j->in[i], j->count * j0.in_elem[i]);
+        off += j->count * j0.in_elem[i];
+      }
+    }
+    const size_t ho = call.out(total * j0.out_elem);
+    call.upload();
+    void* dev_in[2] = {j0.nin > 0 ? call.ptr<char>(hin[0]) : nullptr, j0.nin > 1 ? call.ptr<char>(hin[1]) : nullptr};
+    ck(engine_call(j0, dev_in, total, call.ptr<char>(ho)));
+    size_t off = 0;
+    for (const Job* j : g) {
+      call.download(ho, j->out, j->count * j0.out_elem, off);
+      off += j->count * j0.out_elem;
+    }
+    call.finish();
+    Runtime& rt = Runtime::get();
+    std::lock_guard<std::mutex> lk(rt.qmu);
+    rt.stats.device_calls += 1;
+    rt.stats.max_calls_per_device_call = std::max<uint64_t>(rt.stats.max_calls_per_device_call, g.size());
+  } catch (...) {
+    if (g.size() == 1) {
+      g[0]->err = std::current_exception();
+      return;
+    }
+    for (Job* j : g) run_group(std::vector<Job*>{j});
+  }
+}
+
+// Queues `j` and returns once it ran; rethrows its error.
+void submit(Job& j) {
+  Runtime& rt = Runtime::get();
+  std::unique_lock<std::mutex> lk(rt.qmu);
+  rt.pending.push_back(&j);
+  rt.stats.calls += 1;
+  while (!j.done) {
+    if (!rt.running && !rt.hold) {
+      rt.running = true;
+      std::vector<Job*> take;
+      take.swap(rt.pending);
+      lk.unlock();
+      std::map<std::tuple<int, heu_key*, uint32_t>, std::vector<Job*>> groups;
+      std::vector<std::tuple<int, heu_key*, uint32_t>> order;
+      for (Job* t : take) {
+        auto k = std::make_tuple(t->op, t->key, t->param);
+        auto& v = groups[k];
+        if (v.empty()) order.push_back(k);
+        v.push_back(t);
+      }
+      for (auto& k : order) run_group(groups[k]);
+      lk.lock();
+      for (Job* t : take) t->done = true;
+      rt.running = false;
+      rt.qcv.notify_all();
+    } else {
+      rt.qcv.wait(lk);
+    }
+  }
+  if (j.err) std::rethrow_exception(j.err);
+}
+
+Job make_job(int op, heu_key* key, uint32_t param, size_t count, void* out, size_t out_elem) {
+  Job j;
+  j.op = op;
+  j.key = key;
+  j.param = param;
+  j.count = count;
+  j.out = out;
+  j.out_elem = out_elem;
+  return j;
+}
+
+void add_in(Job& j, const void* p, size_t elem) {
+  j.in[j.nin] = p;
+  j.in_elem[j.nin] = elem;
+  j.nin += 1;
+}
+
 }  // namespace
+
+SpiStats GetSpiStats() {
+  Runtime& rt = Runtime::get();
+  std::lock_guard<std::mutex> lk(rt.qmu);
+  return rt.stats;
+}
+
+void HoldSpiQueue(bool hold) {
+  Runtime& rt = Runtime::get();
+  std::lock_guard<std::mutex> lk(rt.qmu);
+  rt.hold = hold;
+  rt.qcv.notify_all();
+}
+
+size_t SpiQueueLength() {
+  Runtime& rt = Runtime::get();
+  std::lock_guard<std::mutex> lk(rt.qmu);
+  return rt.pending.size();
+}
 
 struct DeviceKey {
   heu_key* key = nullptr;
@@ -422,55 +592,49 @@ std::vector<Ciphertext> add_ct_pt(const DeviceKey& dk, const BigInt& bound, Cons
   auto ca = pack(dk, a);
   PtBatch pb = pack_pts(dk, b, &bound, negate_b);
   std::vector<uint32_t> res(ca.size());
+  if (pb.small) {
+    Job j = make_job(kJobAddI64, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
+    add_in(j, ca.data(), dk.ct_words * 4);
+    add_in(j, pb.i64.data(), 8);
+    submit(j);
+    return unpack(dk, res, cnt);
+  }
+  // plaintexts beyond int64
+  std::vector<uint32_t> rz(cnt, 0u);  // r = 0: h_s^0 = 1
   DeviceCall call;
   size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
-  if (pb.small) {
-    size_t hP = call.in(pb.i64.data(), cnt * 8);
-    call.upload();
-    ck(heu_add_i64(dk.key, call.ptr<uint32_t>(hA), call.ptr<int64_t>(hP), cnt, call.ptr<uint32_t>(hO)));
-  } else {
-    std::vector<uint32_t> rz(cnt, 0u);  // r = 0: h_s^0 = 1
-    size_t hP = call.in(pb.rows.data(), pb.rows.size() * 4), hR = call.in(rz.data(), cnt * 4);
-    size_t hG = call.out(res.size() * 4);
-    call.upload();
-    ck(heu_encrypt(dk.key, call.ptr<uint32_t>(hP), pb.rows.data(), pb.words, call.ptr<uint32_t>(hR), 1, cnt,
-                   call.ptr<uint32_t>(hG)));
-    ck(heu_add(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hG), cnt, call.ptr<uint32_t>(hO)));
-  }
+  size_t hP = call.in(pb.rows.data(), pb.rows.size() * 4), hR = call.in(rz.data(), cnt * 4);
+  size_t hG = call.out(res.size() * 4);
+  call.upload();
+  ck(heu_encrypt(dk.key, call.ptr<uint32_t>(hP), pb.rows.data(), pb.words, call.ptr<uint32_t>(hR), 1, cnt,
+                 call.ptr<uint32_t>(hG)));
+  ck(heu_add(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hG), cnt, call.ptr<uint32_t>(hO)));
   call.download(hO, res.data(), res.size() * 4);
   call.finish();
   return unpack(dk, res, cnt);
 }
 
-template <class F>
-std::vector<Ciphertext> ct_unary(const DeviceKey& dk, ConstSpan<Ciphertext> a, F&& op) {
+std::vector<Ciphertext> ct_unary(const DeviceKey& dk, ConstSpan<Ciphertext> a, int op) {
   const size_t cnt = a.size();
   if (cnt == 0) return {};
   auto ca = pack(dk, a);
   std::vector<uint32_t> res(ca.size());
-  DeviceCall call;
-  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
-  call.upload();
-  ck(op(call.ptr<uint32_t>(hA), cnt, call.ptr<uint32_t>(hO)));
-  call.download(hO, res.data(), res.size() * 4);
-  call.finish();
+  Job j = make_job(op, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
+  add_in(j, ca.data(), dk.ct_words * 4);
+  submit(j);
   return unpack(dk, res, cnt);
 }
 
-template <class F>
-std::vector<Ciphertext> ct_binary(const DeviceKey& dk, ConstSpan<Ciphertext> a, ConstSpan<Ciphertext> b, F&& op) {
+std::vector<Ciphertext> ct_binary(const DeviceKey& dk, ConstSpan<Ciphertext> a, ConstSpan<Ciphertext> b, int op) {
   check_same(a.size(), b.size());
   const size_t cnt = a.size();
   if (cnt == 0) return {};
   auto ca = pack(dk, a), cb = pack(dk, b);
   std::vector<uint32_t> res(ca.size());
-  DeviceCall call;
-  size_t hA = call.in(ca.data(), ca.size() * 4), hB = call.in(cb.data(), cb.size() * 4);
-  size_t hO = call.out(res.size() * 4);
-  call.upload();
-  ck(op(call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hB), cnt, call.ptr<uint32_t>(hO)));
-  call.download(hO, res.data(), res.size() * 4);
-  call.finish();
+  Job j = make_job(op, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
+  add_in(j, ca.data(), dk.ct_words * 4);
+  add_in(j, cb.data(), dk.ct_words * 4);
+  submit(j);
   return unpack(dk, res, cnt);
 }
 
@@ -552,12 +716,9 @@ std::vector<Ciphertext> Encryptor::EncryptZero(int64_t size) const {
   if (cnt == 0) return {};
   auto r = random_r(dk, cnt);
   std::vector<uint32_t> res(cnt * dk.ct_words);
-  DeviceCall call;
-  size_t hR = call.in(r.data(), r.size() * 4), hO = call.out(res.size() * 4);
-  call.upload();
-  ck(heu_randomize(dk.key, nullptr, call.ptr<uint32_t>(hR), dk.r_words, cnt, call.ptr<uint32_t>(hO)));
-  call.download(hO, res.data(), res.size() * 4);
-  call.finish();
+  Job j = make_job(kJobEncZero, dk.key, dk.r_words, cnt, res.data(), dk.ct_words * 4);
+  add_in(j, r.data(), dk.r_words * 4);
+  submit(j);
   return unpack(dk, res, cnt);
 }
 
@@ -568,19 +729,20 @@ std::vector<Ciphertext> Encryptor::EncryptWithR(ConstSpan<Plaintext> pts, const 
   if (r_words == 0 || !r) throw_msg("EncryptWithR: empty randomness");
   PtBatch pb = pack_pts(dk, pts, &pk_.n_half_, false);
   std::vector<uint32_t> res(cnt * dk.ct_words);
+  if (pb.small) {
+    Job j = make_job(kJobEncI64, dk.key, (uint32_t)r_words, cnt, res.data(), dk.ct_words * 4);
+    add_in(j, pb.i64.data(), 8);
+    add_in(j, r, r_words * 4);
+    submit(j);
+    return unpack(dk, res, cnt);
+  }
+  // plaintexts beyond int64
   DeviceCall call;
   size_t hR = call.in(r, cnt * r_words * 4), hO = call.out(res.size() * 4);
-  if (pb.small) {
-    size_t hM = call.in(pb.i64.data(), cnt * 8);
-    call.upload();
-    ck(heu_encrypt_i64(dk.key, call.ptr<int64_t>(hM), call.ptr<uint32_t>(hR), (uint32_t)r_words, cnt,
-                       call.ptr<uint32_t>(hO)));
-  } else {
-    size_t hM = call.in(pb.rows.data(), pb.rows.size() * 4);
-    call.upload();
-    ck(heu_encrypt(dk.key, call.ptr<uint32_t>(hM), pb.rows.data(), pb.words, call.ptr<uint32_t>(hR),
-                   (uint32_t)r_words, cnt, call.ptr<uint32_t>(hO)));
-  }
+  size_t hM = call.in(pb.rows.data(), pb.rows.size() * 4);
+  call.upload();
+  ck(heu_encrypt(dk.key, call.ptr<uint32_t>(hM), pb.rows.data(), pb.words, call.ptr<uint32_t>(hR),
+                 (uint32_t)r_words, cnt, call.ptr<uint32_t>(hO)));
   call.download(hO, res.data(), res.size() * 4);
   call.finish();
   return unpack(dk, res, cnt);
@@ -609,12 +771,9 @@ std::pair<std::vector<Ciphertext>, std::vector<std::string>> Encryptor::EncryptW
   if (cnt == 0) return {cts, audit};
   std::vector<uint32_t> rn(cnt * dk.ct_words);
   {
-    DeviceCall call;
-    size_t hR = call.in(r.data(), r.size() * 4), hO = call.out(rn.size() * 4);
-    call.upload();
-    ck(heu_randomize(dk.key, nullptr, call.ptr<uint32_t>(hR), dk.r_words, cnt, call.ptr<uint32_t>(hO)));
-    call.download(hO, rn.data(), rn.size() * 4);
-    call.finish();
+    Job j = make_job(kJobEncZero, dk.key, dk.r_words, cnt, rn.data(), dk.ct_words * 4);
+    add_in(j, r.data(), dk.r_words * 4);
+    submit(j);
   }
   for (size_t i = 0; i < cnt; ++i) {
     std::vector<uint32_t> one(rn.begin() + i * dk.ct_words, rn.begin() + (i + 1) * dk.ct_words);
@@ -650,12 +809,9 @@ void Decryptor::Decrypt(ConstSpan<Ciphertext> in_cts, Span<Plaintext> out_pts) c
   auto ca = pack(dk, in_cts);
   const uint32_t ow = dk.n_words;  // |m| <= n/2 always fits n_words in two's complement
   std::vector<uint32_t> res(cnt * ow);
-  DeviceCall call;
-  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
-  call.upload();
-  ck(heu_decrypt(dk.key, call.ptr<uint32_t>(hA), cnt, call.ptr<uint32_t>(hO), ow, nullptr));
-  call.download(hO, res.data(), res.size() * 4);
-  call.finish();
+  Job j = make_job(kJobDecrypt, dk.key, ow, cnt, res.data(), ow * 4);
+  add_in(j, ca.data(), dk.ct_words * 4);
+  submit(j);
   for (size_t i = 0; i < cnt; ++i) *out_pts[i] = BigInt::FromWords(res.data() + i * ow, ow, true);
 }
 
@@ -686,21 +842,16 @@ void Evaluator::Randomize(Span<Ciphertext> ct) const {
   auto ca = pack(dk, ct);
   auto r = random_r(dk, cnt);
   std::vector<uint32_t> res(ca.size());
-  DeviceCall call;
-  size_t hA = call.in(ca.data(), ca.size() * 4), hR = call.in(r.data(), r.size() * 4);
-  size_t hO = call.out(res.size() * 4);
-  call.upload();
-  ck(heu_randomize(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hR), dk.r_words, cnt, call.ptr<uint32_t>(hO)));
-  call.download(hO, res.data(), res.size() * 4);
-  call.finish();
+  Job j = make_job(kJobRandomize, dk.key, dk.r_words, cnt, res.data(), dk.ct_words * 4);
+  add_in(j, ca.data(), dk.ct_words * 4);
+  add_in(j, r.data(), dk.r_words * 4);
+  submit(j);
   write_back(ct, unpack(dk, res, cnt));
 }
 
 std::vector<Ciphertext> Evaluator::Add(ConstSpan<Ciphertext> a, ConstSpan<Ciphertext> b) const {
   const DeviceKey& dk = dev(pk_);
-  return ct_binary(dk, a, b, [&](const uint32_t* x, const uint32_t* y, size_t n, uint32_t* o) {
-    return heu_add(dk.key, x, y, n, o);
-  });
+  return ct_binary(dk, a, b, kJobAdd);
 }
 
 std::vector<Ciphertext> Evaluator::Add(ConstSpan<Ciphertext> a, ConstSpan<Plaintext> b) const {
@@ -733,9 +884,7 @@ void Evaluator::AddInplace(Span<Plaintext> a, ConstSpan<Plaintext> b) const {
 // Sub(p, ct) = Add(Negate(ct), p).
 std::vector<Ciphertext> Evaluator::Sub(ConstSpan<Ciphertext> a, ConstSpan<Ciphertext> b) const {
   const DeviceKey& dk = dev(pk_);
-  return ct_binary(dk, a, b, [&](const uint32_t* x, const uint32_t* y, size_t n, uint32_t* o) {
-    return heu_sub(dk.key, x, y, n, o);
-  });
+  return ct_binary(dk, a, b, kJobSub);
 }
 
 std::vector<Ciphertext> Evaluator::Sub(ConstSpan<Ciphertext> a, ConstSpan<Plaintext> b) const {
@@ -781,31 +930,31 @@ std::vector<Ciphertext> Evaluator::Mul(ConstSpan<Ciphertext> a, ConstSpan<Plaint
   }
   auto ca = pack(dk, a);
   std::vector<uint32_t> res(ca.size());
-  DeviceCall call;
-  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
-  std::vector<int64_t> k64;
-  std::vector<uint32_t> mag;
-  std::vector<uint8_t> neg;
   if (small) {
+    std::vector<int64_t> k64;
     k64.reserve(cnt);
     for (const Plaintext* p : b) k64.push_back(p->ToInt64());
-    size_t hK = call.in(k64.data(), cnt * 8);
-    call.upload();
-    ck(heu_mul_i64(dk.key, call.ptr<uint32_t>(hA), call.ptr<int64_t>(hK), cnt, call.ptr<uint32_t>(hO)));
-  } else {
-    const uint32_t kw = (uint32_t)((bits + 31) / 32);
-    mag.assign(cnt * kw, 0u);
-    neg.resize(cnt);
-    for (size_t i = 0; i < cnt; ++i) {
-      bn_check(BN_bn2lebinpad(b[i]->bn(), reinterpret_cast<unsigned char*>(mag.data() + i * kw), (int)kw * 4) >= 0,
-               "bn2lebinpad");
-      neg[i] = b[i]->IsNegative() ? 1 : 0;
-    }
-    size_t hK = call.in(mag.data(), mag.size() * 4), hS = call.in(neg.data(), cnt);
-    call.upload();
-    ck(heu_mul(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hK), call.ptr<uint8_t>(hS), kw, cnt,
-               call.ptr<uint32_t>(hO)));
+    Job j = make_job(kJobMulI64, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
+    add_in(j, ca.data(), dk.ct_words * 4);
+    add_in(j, k64.data(), 8);
+    submit(j);
+    return unpack(dk, res, cnt);
   }
+  // scalars beyond int64
+  const uint32_t kw = (uint32_t)((bits + 31) / 32);
+  std::vector<uint32_t> mag(cnt * kw, 0u);
+  std::vector<uint8_t> neg(cnt);
+  for (size_t i = 0; i < cnt; ++i) {
+    bn_check(BN_bn2lebinpad(b[i]->bn(), reinterpret_cast<unsigned char*>(mag.data() + i * kw), (int)kw * 4) >= 0,
+             "bn2lebinpad");
+    neg[i] = b[i]->IsNegative() ? 1 : 0;
+  }
+  DeviceCall call;
+  size_t hA = call.in(ca.data(), ca.size() * 4), hO = call.out(res.size() * 4);
+  size_t hK = call.in(mag.data(), mag.size() * 4), hS = call.in(neg.data(), cnt);
+  call.upload();
+  ck(heu_mul(dk.key, call.ptr<uint32_t>(hA), call.ptr<uint32_t>(hK), call.ptr<uint8_t>(hS), kw, cnt,
+             call.ptr<uint32_t>(hO)));
   call.download(hO, res.data(), res.size() * 4);
   call.finish();
   return unpack(dk, res, cnt);
@@ -831,7 +980,7 @@ void Evaluator::MulInplace(Span<Plaintext> a, ConstSpan<Plaintext> b) const {
 
 std::vector<Ciphertext> Evaluator::Negate(ConstSpan<Ciphertext> a) const {
   const DeviceKey& dk = dev(pk_);
-  return ct_unary(dk, a, [&](const uint32_t* x, size_t n, uint32_t* o) { return heu_negate(dk.key, x, n, o); });
+  return ct_unary(dk, a, kJobNegate);
 }
 
 void Evaluator::NegateInplace(Span<Ciphertext> a) const { write_back(a, Negate(ConstSpan<Ciphertext>(a.data(), a.size()))); }

@@ -10,6 +10,12 @@
 // heu/library/algorithms/util/spi_traits.h:22-26.  Every batch call runs one
 // engine entry point on the device; errors throw std::runtime_error (HEU
 // throws yacl::Exception).
+//
+// Methods are const and re-entrant: HEU calls them from its worker threads,
+// one chunk per thread (numpy/encryptor.cc:28, numpy/evaluator.cc:124), and
+// scalar calls arrive as length-1 spans.  Elementwise calls coalesce: a call
+// that finds the device busy queues, and the next thread to take the device
+// runs every queued call of the same operation and key as one engine call.
 #pragma once
 
 #include <openssl/bn.h>
@@ -209,6 +215,18 @@ class Evaluator {
  private:
   PublicKey pk_;
 };
+
+// Coalescing counters of this process: span calls submitted, engine calls that
+// ran them, and the most calls one engine call served.
+struct SpiStats {
+  uint64_t calls = 0, device_calls = 0, max_calls_per_device_call = 0;
+};
+SpiStats GetSpiStats();
+// Diagnostics: while held, queued calls wait (none takes the device), so a
+// test can line up calls and release them into one engine call.
+void HoldSpiQueue(bool hold);
+// Calls waiting in the queue right now.
+size_t SpiQueueLength();
 
 }  // namespace paillier_hip
 }  // namespace heu_amd

@@ -8,12 +8,14 @@
 // iff every check held.
 #include <openssl/bn.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <functional>
 #include <limits>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "paillier_hip.h"
@@ -376,6 +378,90 @@ void VectorizedClosedForms(const Keys& k, size_t count) {
   EXPECT_THROW(evaluator.Add(cs(pc), cs(shorter)));
 }
 
+// HEU's numpy layer calls the span methods from its worker threads, one chunk
+// per thread (numpy/encryptor.cc:28, numpy/evaluator.cc:124).  Calls lined up
+// while the queue is held run as one engine call per (operation, key), and
+// every thread gets what it gets alone.  One Mul chunk holds a ciphertext 0
+// times -1 (no inverse): only that call throws.
+void ConcurrentChunks(const Keys& k, const Keys& k2) {
+  Encryptor enc(k.pk), enc2(k2.pk);
+  Decryptor dec(k.pk, k.sk);
+  Evaluator ev(k.pk), ev2(k2.pk);
+  constexpr int T = 8;
+  constexpr size_t kPer = 33;
+  std::mt19937_64 g(7);
+  std::vector<std::vector<BigInt>> m(T), w(T);
+  std::vector<std::vector<Ciphertext>> c(T), c2(T);
+  for (int t = 0; t < T; ++t) {
+    for (size_t i = 0; i < kPer; ++i) {
+      m[t].push_back(BigInt((int64_t)g()));
+      w[t].push_back(BigInt((int64_t)(int32_t)g()));
+    }
+    c[t] = enc.Encrypt(cs(cptrs(m[t])));
+    c2[t] = enc2.Encrypt(cs(cptrs(m[t])));
+  }
+  // the poisoned Mul chunk: element 5 is the ciphertext 0, times -1
+  std::vector<Ciphertext> bad = c[3];
+  bad[5].c_.assign(bad[5].c_.size(), 0u);
+  std::vector<BigInt> wbad = w[3];
+  wbad[5] = BigInt(-1);
+  // one thread, one call at a time
+  std::vector<std::vector<Ciphertext>> add_want(T), add2_want(T), mul_want(T);
+  std::vector<std::vector<BigInt>> dec_want(T);
+  for (int t = 0; t < T; ++t) {
+    add_want[t] = ev.Add(cs(cptrs(c[t])), cs(cptrs(c[(t + 1) % T])));
+    add2_want[t] = ev2.Add(cs(cptrs(c2[t])), cs(cptrs(c2[(t + 2) % T])));
+    if (t != 3) mul_want[t] = ev.Mul(cs(cptrs(c[t])), cs(cptrs(w[t])));
+    dec_want[t] = dec.Decrypt(cs(cptrs(c[t])));
+  }
+  std::vector<std::vector<Ciphertext>> add_got(T), add2_got(T), mul_got(T);
+  std::vector<std::vector<BigInt>> dec_got(T);
+  std::vector<int> threw(T, 0);
+  const SpiStats s0 = GetSpiStats();
+  HoldSpiQueue(true);
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t) {
+    th.emplace_back([&, t] { add_got[t] = ev.Add(cs(cptrs(c[t])), cs(cptrs(c[(t + 1) % T]))); });
+    th.emplace_back([&, t] { add2_got[t] = ev2.Add(cs(cptrs(c2[t])), cs(cptrs(c2[(t + 2) % T]))); });
+    th.emplace_back([&, t] {
+      try {
+        mul_got[t] = t == 3 ? ev.Mul(cs(cptrs(bad)), cs(cptrs(wbad))) : ev.Mul(cs(cptrs(c[t])), cs(cptrs(w[t])));
+      } catch (const std::exception&) {
+        threw[t] = 1;
+      }
+    });
+    th.emplace_back([&, t] { dec_got[t] = dec.Decrypt(cs(cptrs(c[t]))); });
+  }
+  const auto deadline = std::chrono::steady_clock::now() + std::chrono::seconds(30);
+  while (SpiQueueLength() < (size_t)(4 * T) && std::chrono::steady_clock::now() < deadline)
+    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+  EXPECT_EQ(SpiQueueLength(), (size_t)(4 * T));
+  HoldSpiQueue(false);
+  for (auto& x : th) x.join();
+  const SpiStats s1 = GetSpiStats();
+  std::printf("  %llu calls -> %llu engine calls (largest group %llu calls)\n",
+              (unsigned long long)(s1.calls - s0.calls), (unsigned long long)(s1.device_calls - s0.device_calls),
+              (unsigned long long)s1.max_calls_per_device_call);
+  EXPECT_EQ(s1.calls - s0.calls, (uint64_t)(4 * T));
+  // Add, Add on the second key, Decrypt: one call each; Mul: the failed group
+  // call, then its 8 calls one by one (7 succeed)
+  EXPECT_EQ(s1.device_calls - s0.device_calls, (uint64_t)(3 + T - 1));
+  EXPECT_TRUE(s1.max_calls_per_device_call >= (uint64_t)T);
+  for (int t = 0; t < T; ++t) {
+    EXPECT_EQ(add_got[t].size(), kPer);
+    for (size_t i = 0; i < add_got[t].size(); ++i) EXPECT_TRUE(add_got[t][i].c_ == add_want[t][i].c_);
+    for (size_t i = 0; i < add2_got[t].size(); ++i) EXPECT_TRUE(add2_got[t][i].c_ == add2_want[t][i].c_);
+    EXPECT_EQ(add2_got[t].size(), kPer);
+    EXPECT_TRUE(dec_got[t] == dec_want[t]);
+    EXPECT_TRUE(dec_got[t] == m[t]);
+    EXPECT_EQ(threw[t], t == 3 ? 1 : 0);
+    if (t != 3) {
+      EXPECT_EQ(mul_got[t].size(), kPer);
+      for (size_t i = 0; i < mul_got[t].size(); ++i) EXPECT_TRUE(mul_got[t][i].c_ == mul_want[t][i].c_);
+    }
+  }
+}
+
 }  // namespace
 
 int main() {
@@ -401,6 +487,7 @@ int main() {
   run("KeyGenerator.Generate(3072)", [&] { KeyGenerator::Generate(3072, &k3072.sk, &k3072.pk); });
   run("ZPaillierTest.MinMaxDecrypt(3072)", [&] { MinMaxDecrypt(k3072); });
   run("Vectorized.ClosedForms(3072, 65)", [&] { VectorizedClosedForms(k3072, 65); });
+  run("Concurrent chunks coalesce (2048 + 3072 keys)", [&] { ConcurrentChunks(seeded, k3072); });
   run("Decryptor rejects an unpaired key", [&] { EXPECT_THROW(Decryptor(k2048.pk, k3072.sk)); });
   if (g_failures) {
     std::printf("%d FAILURE(S)\n", g_failures);
