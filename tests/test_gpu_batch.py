@@ -217,3 +217,49 @@ def test_context_outlives_its_objects(eng):
     assert np.array_equal(host(d).view(np.int64).ravel(), m)
     b.close()
     k.close()  # the last holder: frees the context
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_stages_with_barriers(eng, ctx, keys, graph):
+    # three stages over both key sizes, each stage reading the previous one's
+    # outputs: encrypt | decrypt, ct x k, dot | decrypt(ct x k).  Each stage's
+    # shapes run as parallel lanes; the barrier must join them all.
+    b = eng.Batch(ctx, graph=graph)
+    recs = []
+    for bits in (2048, 3072):
+        kd, gk, ok = keys[bits]
+        for m, k, r in vectors(eng, bits + 5, [300, 65, 1]):
+            recs.append([bits, m, k, r, b.encrypt_i64(gk, m, r)])
+    b.barrier()
+    for rec in recs:
+        bits, m, k, r, ct = rec
+        gk = keys[bits][1]
+        offs = np.array([0, len(m) // 2, len(m)], dtype=np.int64)
+        rec += [b.decrypt(gk, ct, out_words=2), b.mul_i64(gk, ct, k), b.dot_i64(gk, ct, k, offs), offs]
+    b.barrier()
+    for rec in recs:
+        bits, m, k, r, ct, dec, mul, dot, offs = rec
+        rec.append(b.decrypt(keys[bits][1], mul, out_words=3))
+    st = b.stats()
+    print(f"stages: graph={graph} {st}", flush=True)
+    want = []
+    for bits, m, k, r, ct, dec, mul, dot, offs, dmul in recs:
+        gk = keys[bits][1]
+        c = gk.encrypt_i64(m, r)
+        want.append((host(c), host(gk.mul_i64(c, k)), host(gk.dot_i64(c, k, offs))))
+    ctx.sync()
+    for rep in range(3):
+        for rec in recs:
+            for t in rec[4:8] + rec[9:]:
+                t.zero_()
+        print(f"stages: graph={graph} run {rep}", flush=True)
+        b.run()
+        ctx.sync()
+        for (bits, m, k, r, ct, dec, mul, dot, offs, dmul), (wc, wm, wd) in zip(recs, want):
+            assert np.array_equal(host(ct), wc), (rep, bits, "encrypt")
+            assert np.array_equal(host(dec).view(np.int64).ravel(), m), (rep, bits, "decrypt")
+            assert np.array_equal(host(mul), wm), (rep, bits, "mul")
+            assert np.array_equal(host(dot), wd), (rep, bits, "dot")
+            got = [eng.twos_to_int(row) for row in host(dmul)]
+            assert got == [int(a) * int(c) for a, c in zip(m, k)], (rep, bits, "decrypt of mul")
+    b.close()
