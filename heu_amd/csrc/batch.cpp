@@ -113,15 +113,27 @@ WorkItem make_item(int op, long count, const heu_key* key) {
   return it;
 }
 
-// Batch inversion (Montgomery's trick) of the selected elements of `in` into
-// `out` (may alias), as steps.
-void plan_invert(Builder& b, heu_key* key, const uint32_t* in, const uint8_t* sel, long count, uint32_t* out) {
+// Batch inversion (Montgomery's trick) of the elements of `in` into `out`
+// (may alias), as steps: all of them, or (`selected`) those flagged in `sel`.
+// The choice is the caller's, not sel's value: in the sizing pass workspace
+// pointers are offsets, and the first one is null.
+void plan_invert(Builder& b, heu_key* key, const uint32_t* in, const uint8_t* sel, bool selected, long count,
+                 uint32_t* out) {
   if (count <= 0) return;
   const Ops* o = key->ops;
   const ModArgs mod = key->mod_n2();
   const size_t NLB = (size_t)key->nl_n * 4;
   const std::vector<long> cnt = inv_levels(count);
   const int levels = (int)cnt.size() - 1;
+  // selected elements (in place) run from a packed index list
+  heu_hip::InvSel s0{selected ? sel : nullptr, nullptr, nullptr};
+  if (selected && in == out) {
+    uint32_t* idx = b.alloc<uint32_t>((size_t)count * 4);
+    uint32_t* nsel = b.alloc<uint32_t>(4);
+    s0 = heu_hip::InvSel{nullptr, idx, nsel};
+    b.then([=](hipStream_t st) { o->inv_compact(st, sel, count, idx, nsel); }, 2);
+  }
+  const heu_hip::InvSel none{nullptr, nullptr, nullptr};
   std::vector<uint32_t*> pre(levels + 1, nullptr), tot(levels + 1), inv(levels + 1);
   for (int l = 0; l <= levels; ++l) {
     if (l < levels) pre[l] = b.alloc<uint32_t>(cnt[l] * NLB);
@@ -130,10 +142,11 @@ void plan_invert(Builder& b, heu_key* key, const uint32_t* in, const uint8_t* se
   }
   for (int l = 0; l < levels; ++l) {
     const uint32_t* src = l == 0 ? in : tot[l];
-    const uint8_t* s = l == 0 ? sel : nullptr;
+    const heu_hip::InvSel s = l == 0 ? s0 : none;
     uint32_t *p = pre[l], *t = tot[l + 1];
     const long n = cnt[l];
-    b.then([=](hipStream_t st) { o->inv_prefix(st, mod, l == 0, src, s, n, kInvChunk, p, t); });
+    const int chunk = inv_chunk(l);
+    b.then([=](hipStream_t st) { o->inv_prefix(st, mod, l == 0, src, s, n, chunk, p, t); });
   }
   const uint32_t* n2w = key->n2_words.as<uint32_t>();
   uint32_t *top_inv = inv[levels], *top_tot = tot[levels], *err = key->ctx->d_err;
@@ -143,11 +156,12 @@ void plan_invert(Builder& b, heu_key* key, const uint32_t* in, const uint8_t* se
   }, 2);
   for (int l = levels - 1; l >= 0; --l) {
     const uint32_t* src = l == 0 ? in : tot[l];
-    const uint8_t* s = l == 0 ? sel : nullptr;
+    const heu_hip::InvSel s = l == 0 ? s0 : none;
     uint32_t* dst = l == 0 ? out : inv[l];
     uint32_t *p = pre[l], *ti = inv[l + 1];
     const long n = cnt[l];
-    b.then([=](hipStream_t st) { o->inv_down(st, mod, l == 0, src, s, n, kInvChunk, p, ti, dst); });
+    const int chunk = inv_chunk(l);
+    b.then([=](hipStream_t st) { o->inv_down(st, mod, l == 0, src, s, n, chunk, p, ti, dst); });
   }
 }
 
@@ -219,7 +233,7 @@ void plan_segmented(Builder& b, heu_key* key, const heu_task& t, WorkList& wl) {
     uint32_t* nz = b.alloc<uint32_t>(segs * (size_t)key->ct_words * 4);
     uint32_t *pos = top.pos, *neg = top.neg;
     b.then([=](hipStream_t st) { o->seg_finish(st, mod, neg, nullptr, (long)segs, 1, nz); });
-    plan_invert(b, key, nz, nullptr, (long)segs, nz);
+    plan_invert(b, key, nz, nullptr, false, (long)segs, nz);
     b.then([=](hipStream_t st) { o->mont_times_z(st, mod, pos, nz, (long)segs, out); });
   }
 }
@@ -308,7 +322,7 @@ int plan_stage(Builder& b, heu_ctx* ctx, const heu_task* tasks, size_t ntasks, u
         wl.items.emplace_back(unit_cost(it.op), it);
         uint32_t* out = static_cast<uint32_t*>(t.out);
         b.cur = &wl.follow;
-        plan_invert(b, key, out, neg, n, out);
+        plan_invert(b, key, out, neg, true, n, out);
         b.cur = nullptr;
         break;
       }

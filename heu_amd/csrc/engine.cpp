@@ -73,7 +73,10 @@ void ctx_release(heu_ctx* ctx) {
 
 std::vector<long> inv_levels(long count) {
   std::vector<long> cnt{count};
-  while (cnt.size() < 2 || cnt.back() > 1) cnt.push_back((cnt.back() + kInvChunk - 1) / kInvChunk);
+  while (cnt.size() < 2 || cnt.back() > 1) {
+    const long c = inv_chunk(cnt.size() - 1);
+    cnt.push_back((cnt.back() + c - 1) / c);
+  }
   return cnt;
 }
 
@@ -306,7 +309,7 @@ int check_key(heu_key* key) {
 size_t inv_scratch_bytes(const heu_key* key, long count) {
   const size_t NLB = (size_t)key->nl_n * 4;
   auto cnt = inv_levels(count);
-  size_t b = 0;
+  size_t b = (((size_t)count * 4 + 255) & ~(size_t)255) + 256;  // index list and its count
   for (size_t l = 0; l < cnt.size(); ++l) {
     size_t part = (cnt[l] * NLB + 255) & ~(size_t)255;
     b += (l + 1 < cnt.size() ? 3 : 2) * part;
@@ -323,6 +326,9 @@ int batch_invert(heu_key* key, const uint32_t* in, const uint8_t* sel, long coun
   std::vector<long> cnt = inv_levels(count);
   const int levels = (int)cnt.size() - 1;  // cnt[levels] == 1
   Scratch sc(ctx, base);
+  // selected elements (always in place here) run from a packed index list
+  const bool compact = sel != nullptr && in == out;
+  const size_t idx_off = sc.add((size_t)count * 4), nsel_off = sc.add(4);
   std::vector<size_t> pre(levels + 1), tot(levels + 1), inv(levels + 1);
   for (int l = 0; l <= levels; ++l) {
     if (l < levels) pre[l] = sc.add(cnt[l] * NLB);
@@ -332,11 +338,17 @@ int batch_invert(heu_key* key, const uint32_t* in, const uint8_t* sel, long coun
   int rc = sc.commit();
   if (rc) return rc;
   hipStream_t st = ctx->stream;
+  heu_hip::InvSel s0{sel, nullptr, nullptr};
+  if (compact) {
+    s0 = heu_hip::InvSel{nullptr, sc.at<uint32_t>(idx_off), sc.at<uint32_t>(nsel_off)};
+    LAUNCH(ctx, "inv_compact", o->inv_compact(st, sel, count, sc.at<uint32_t>(idx_off), sc.at<uint32_t>(nsel_off)));
+  }
+  const heu_hip::InvSel none{nullptr, nullptr, nullptr};
   // up: level l reduces cnt[l] values into cnt[l+1] totals
   for (int l = 0; l < levels; ++l) {
     const uint32_t* src = l == 0 ? in : sc.at<uint32_t>(tot[l]);
-    LAUNCH(ctx, "inv_prefix", o->inv_prefix(st, mod, l == 0, src, l == 0 ? sel : nullptr, cnt[l], kInvChunk, sc.at<uint32_t>(pre[l]),
-                  sc.at<uint32_t>(tot[l + 1])));
+    LAUNCH(ctx, "inv_prefix", o->inv_prefix(st, mod, l == 0, src, l == 0 ? s0 : none, cnt[l], inv_chunk(l),
+                                            sc.at<uint32_t>(pre[l]), sc.at<uint32_t>(tot[l + 1])));
   }
   // the single top value
   HIP_TRY(hipMemcpyAsync(sc.at<uint32_t>(inv[levels]), sc.at<uint32_t>(tot[levels]), NLB, hipMemcpyDeviceToDevice, st));
@@ -345,8 +357,8 @@ int batch_invert(heu_key* key, const uint32_t* in, const uint8_t* sel, long coun
   for (int l = levels - 1; l >= 0; --l) {
     const uint32_t* src = l == 0 ? in : sc.at<uint32_t>(tot[l]);
     uint32_t* dst = l == 0 ? out : sc.at<uint32_t>(inv[l]);
-    LAUNCH(ctx, "inv_down", o->inv_down(st, mod, l == 0, src, l == 0 ? sel : nullptr, cnt[l], kInvChunk, sc.at<uint32_t>(pre[l]),
-                sc.at<uint32_t>(inv[l + 1]), dst));
+    LAUNCH(ctx, "inv_down", o->inv_down(st, mod, l == 0, src, l == 0 ? s0 : none, cnt[l], inv_chunk(l),
+                                        sc.at<uint32_t>(pre[l]), sc.at<uint32_t>(inv[l + 1]), dst));
   }
   HIP_TRY(hipGetLastError());
   return HEU_OK;

@@ -53,10 +53,14 @@ struct DevBuf {
   }
 };
 
-constexpr int kInvChunk = 32;
+// Batch inversion (Montgomery's trick): level 0 multiplies the inputs in
+// chunks of kInvChunk (real work, the whole chip); upper levels see few values,
+// where a level's time is its chunk's chain of sequential products, so they use
+// chunks of kInvChunkUpper.
+constexpr int kInvChunk = 32, kInvChunkUpper = 4;
+inline int inv_chunk(size_t level) { return level == 0 ? kInvChunk : kInvChunkUpper; }
 
-// Element counts per level of a batch inversion (Montgomery's trick over
-// chunks of kInvChunk, up to one value).
+// Element counts per level of a batch inversion (up to one value).
 std::vector<long> inv_levels(long count);
 
 // Chunking of segments for a reduction level.  Level 0 reads the inputs in

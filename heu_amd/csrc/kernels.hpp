@@ -102,10 +102,18 @@ typedef void (*PrefixMulFn)(hipStream_t, const ModArgs&, const uint32_t*, const 
 typedef void (*ReduceMontFn)(hipStream_t, const ModArgs&, const uint32_t*, const Chunk*, long, uint32_t*);
 typedef void (*SegFinishFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, int, uint32_t*);
 typedef void (*MontTimesZFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
-typedef void (*InvPrefixFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const u8*, long, int, uint32_t*,
+// Which level-0 elements a batch inversion works on: all of them (both null),
+// those with sel[i] != 0, or positions [0, *nsel) of the index list idx.
+struct InvSel {
+  const u8* sel;
+  const uint32_t* idx;
+  const uint32_t* nsel;
+};
+typedef void (*InvPrefixFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, InvSel, long, int, uint32_t*,
                             uint32_t*);
-typedef void (*InvDownFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const u8*, long, int, uint32_t*,
+typedef void (*InvDownFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, InvSel, long, int, uint32_t*,
                           const uint32_t*, uint32_t*);
+typedef void (*InvCompactFn)(hipStream_t, const u8*, long, uint32_t*, uint32_t*);
 typedef void (*InvSingleFn)(hipStream_t, const ModArgs&, const uint32_t*, uint32_t*, uint32_t*);
 typedef void (*MulConstFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
 typedef void (*DecHalfFn)(hipStream_t, const DecArgs&, const uint32_t*, long, uint32_t*, uint32_t*);
@@ -155,6 +163,7 @@ struct Ops {
   MontTimesZFn mont_times_z;
   InvPrefixFn inv_prefix;
   InvDownFn inv_down;
+  InvCompactFn inv_compact;
   InvSingleFn inv_single;
   MulConstFn mul_const;
   PrefixMulFn prefix_mul;

@@ -957,8 +957,40 @@ __device__ __forceinline__ bool chunk_selected(const u8* __restrict__ sel, long 
   return any;
 }
 
+// Level-0 selections by index list: the selected elements' positions, in any
+// order (each element's inverse does not depend on how elements are grouped),
+// packed with one atomic per wave.  *nsel must be 0 on entry.  (Templates
+// only so that every translation unit including this header may define them.)
+template <int kUnused = 0>
+__global__ __launch_bounds__(kBlock) void k_inv_compact(const u8* __restrict__ sel, long count, u32* __restrict__ idx,
+                                                         u32* nsel) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  const bool take = i < count && sel[i] != 0;
+  const u64 mask = __builtin_amdgcn_ballot_w64(take);
+  if (mask == 0) return;  // uniform
+  const u32 lane = threadIdx.x & 63u;
+  const u32 first = (u32)__builtin_ctzll(mask);
+  u32 base = 0;
+  if (lane == first) base = atomicAdd(nsel, (u32)__builtin_popcountll(mask));
+  base = (u32)__shfl((int)base, (int)first, 64);
+  if (take) idx[base + (u32)__builtin_popcountll(mask & ((1ull << lane) - 1ull))] = (u32)i;
+}
+
+template <int kUnused = 0>
+__global__ __launch_bounds__(64) void k_zero_u32(u32* p) {
+  if (threadIdx.x == 0) p[0] = 0u;
+}
+
+// Position j of level 0 -> element index; last position that holds one.
+__device__ __forceinline__ long inv_elem(const InvSel& s, long j) { return s.idx ? (long)s.idx[j] : j; }
+__device__ __forceinline__ long inv_last(const InvSel& s, long last) {
+  if (!s.idx) return last;
+  const long n = (long)*s.nsel;
+  return n < last ? n : last;
+}
+
 template <int L, int K, int W, bool ZIN>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_prefix(ModArgs mod, const u32* __restrict__ in, const u8* __restrict__ sel, long count,
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_prefix(ModArgs mod, const u32* __restrict__ in, InvSel s, long count,
                                    int chunk, u32* __restrict__ prefix, u32* __restrict__ totals) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
@@ -966,24 +998,27 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_prefix(M
   auto g = grp_setup<L, K, W>(lds, mod, e);
   const long first = e * chunk;
   if (first >= count) return;
-  const long last = first + chunk < count ? first + chunk : count;
+  long last = first + chunk < count ? first + chunk : count;
+  if constexpr (ZIN) last = inv_last(s, last);  // positions past the list count as 1
   g.set_global(g.A, mod.one);
-  if (chunk_selected<ZIN>(sel, first, last)) {
-    for (long i = first; i < last; ++i) {
+  if (first < last && chunk_selected<ZIN>(s.sel, first, last)) {
+    for (long j = first; j < last; ++j) {
       u32 x[K];
-      if (inv_x<L, K, W, ZIN>(x, g, mod, in, sel, i)) g.mul(x);  // unselected: X = 1
-      g.save(prefix + i * NL, g.A);
+      const long i = ZIN ? inv_elem(s, j) : j;
+      if (inv_x<L, K, W, ZIN>(x, g, mod, in, s.sel, i)) g.mul(x);  // unselected: X = 1
+      g.save(prefix + j * NL, g.A);
     }
   }
   g.save(totals + e * NL, g.A);
 }
 
 // Down pass.  `out` may alias `in`: element i is read (to rebuild X_i) before
-// out[i] is written.  y_i is parked in the spent prefix slot P_(i-1).
+// out[i] is written.  y at position j is parked in the spent prefix slot
+// P_(j-1).  With an index list the inversion is in place (out == in) and
+// elements off the list are left as they are.
 template <int L, int K, int W, bool ZIN>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(ModArgs mod, const u32* in,
-                                                                           const u8* __restrict__ sel, long count,
-                                                                           int chunk, u32* prefix,
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(ModArgs mod, const u32* in, InvSel s,
+                                                                           long count, int chunk, u32* prefix,
                                                                            const u32* __restrict__ tot_inv, u32* out) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
@@ -991,7 +1026,14 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(Mod
   auto g = grp_setup<L, K, W>(lds, mod, e);
   const long first = e * chunk;
   if (first >= count) return;
-  const long last = first + chunk < count ? first + chunk : count;
+  long last = first + chunk < count ? first + chunk : count;
+  if constexpr (ZIN) {
+    if (s.idx) {
+      last = inv_last(s, last);
+      if (first >= last) return;
+    }
+  }
+  const u8* sel = s.sel;
   if (!chunk_selected<ZIN>(sel, first, last)) {
     if constexpr (ZIN) {
       if (out != in)
@@ -1001,7 +1043,8 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(Mod
     return;
   }
   g.set_global(g.A, tot_inv + e * NL);  // A = inv (Montgomery form)
-  for (long i = last - 1; i >= first; --i) {
+  for (long j = last - 1; j >= first; --j) {
+    const long i = ZIN ? inv_elem(s, j) : j;
     if constexpr (ZIN) {
       if (sel && !sel[i]) {
         if (out != in)
@@ -1010,11 +1053,11 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(Mod
       }
     }
     u32* ypark = nullptr;
-    if (i > first) {
+    if (j > first) {
       u32 x[K];
       inv_x<L, K, W, ZIN>(x, g, mod, in, sel, i);  // X_i, read before out[i] is written
       g.set(g.B, x);
-      ypark = prefix + (i - 1) * NL;
+      ypark = prefix + (j - 1) * NL;
       u32 b[K];
       load_r29<L, K>(b, ypark, g.lid);
       g.mul_to(x, g.A, b);  // y_i = X_i^-1 R = inv * P_(i-1)
@@ -1398,17 +1441,25 @@ struct Launch {
   static void mont_times_z(hipStream_t st, const ModArgs& mod, const u32* xm, const u32* yz, long n, u32* out) {
     HEU_LAUNCH((k_mont_times_z<L, K, W>), n, L, st, mod, xm, yz, n, out);
   }
-  static void inv_prefix(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const u8* sel, long n, int chunk,
+  static void inv_prefix(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, InvSel s, long n, int chunk,
                          u32* prefix, u32* totals) {
     long groups = (n + chunk - 1) / chunk;
-    if (zin) HEU_LAUNCH((k_inv_prefix<L, K, W, true>), groups, L, st, mod, in, sel, n, chunk, prefix, totals);
-    else HEU_LAUNCH((k_inv_prefix<L, K, W, false>), groups, L, st, mod, in, sel, n, chunk, prefix, totals);
+    if (zin) HEU_LAUNCH((k_inv_prefix<L, K, W, true>), groups, L, st, mod, in, s, n, chunk, prefix, totals);
+    else HEU_LAUNCH((k_inv_prefix<L, K, W, false>), groups, L, st, mod, in, s, n, chunk, prefix, totals);
   }
-  static void inv_down(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const u8* sel, long n, int chunk,
+  static void inv_down(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, InvSel s, long n, int chunk,
                        u32* prefix, const u32* tot_inv, u32* out) {
     long groups = (n + chunk - 1) / chunk;
-    if (zin) HEU_LAUNCH((k_inv_down<L, K, W, true>), groups, L, st, mod, in, sel, n, chunk, prefix, tot_inv, out);
-    else HEU_LAUNCH((k_inv_down<L, K, W, false>), groups, L, st, mod, in, sel, n, chunk, prefix, tot_inv, out);
+    if (zin) HEU_LAUNCH((k_inv_down<L, K, W, true>), groups, L, st, mod, in, s, n, chunk, prefix, tot_inv, out);
+    else HEU_LAUNCH((k_inv_down<L, K, W, false>), groups, L, st, mod, in, s, n, chunk, prefix, tot_inv, out);
+  }
+  // idx receives the positions of the selected elements and *nsel their count
+  // (cleared first by a kernel, so the pair can be captured in a graph)
+  static void inv_compact(hipStream_t st, const u8* sel, long n, u32* idx, u32* nsel) {
+    hipLaunchKernelGGL(k_zero_u32<0>, dim3(1), dim3(64), 0, st, nsel);
+    if (n > 0)
+      hipLaunchKernelGGL(k_inv_compact<0>, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, sel, n, idx,
+                         nsel);
   }
   static void inv_single(hipStream_t st, const ModArgs& mod, const u32* m_words, u32* xio, u32* err) {
     hipLaunchKernelGGL((k_inv_single<L, K, W>), dim3(1), dim3(64), 0, st, mod, m_words, xio, err);
@@ -1496,6 +1547,7 @@ static Ops make_ops(const WorklistShape& wl_n, const DecShape& dec0, const DecSh
   o.mont_times_z = Launch<SN>::mont_times_z;
   o.inv_prefix = Launch<SN>::inv_prefix;
   o.inv_down = Launch<SN>::inv_down;
+  o.inv_compact = Launch<SN>::inv_compact;
   o.inv_single = Launch<SN>::inv_single;
   o.mul_const = Launch<SN>::mul_const;
   o.prefix_mul = Launch<SN>::prefix_mul;
