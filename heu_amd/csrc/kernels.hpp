@@ -174,6 +174,7 @@ struct Ops {
 
 const Ops* ops_for_key_bits(int key_bits);
 
+constexpr int kBlock = 256;         // threads per workgroup of every group kernel
 constexpr int kPowWindowBits = 4;  // fixed window of k_pow_i64 / dot terms
 constexpr int kDecWindowBits = 5;  // sliding window of the CRT exponentiations
 

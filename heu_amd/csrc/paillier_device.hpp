@@ -14,7 +14,6 @@
 
 namespace heu_hip {
 
-constexpr int kBlock = 256;
 constexpr int kPowWin = kPowWindowBits;
 constexpr int kDecWin = kDecWindowBits;
 
