@@ -1081,110 +1081,135 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(Mod
 }
 
 // ---------------------------------------------------------------------------
-// The one inversion per batch, by binary extended Euclid on wave-wide big
-// integers: a W-word number is spread over the 64 lanes of one wave,
-// WPL = W/64 consecutive words per lane.  Carries and borrows cross lanes by
-// carry lookahead on ballot masks: with G = lanes that generate a carry and
-// P = lanes that pass an incoming one on, bit i of ((G|P) + G) ^ (G|P) ^ G is
-// the carry into lane i, and the 64-bit addition's overflow is the carry out.
+// The one inversion per batch: x^-1 mod m by Bernstein-Yang divsteps (D. J.
+// Bernstein, B.-Y. Yang, "Fast constant-time gcd computation and modular
+// inversion", 2019), 60 divsteps at a time.  One wave holds f, g (the gcd pair,
+// starting at m and x) and d, e (f = d x, g = e x mod m) as 128 signed limbs of
+// 60 bits, two per lane.  A batch runs 60 divsteps on the low 64 bits of f and
+// g (uniform values, the same in every lane), which yield the transition matrix
+// [u v; q r] with |u| + |v|, |q| + |r| <= 2^60; then f, g <- [u v; q r] (f, g)
+// / 2^60 and d, e likewise after adding the multiple k m of m that makes them
+// divisible.  Limbs stay redundant, in [-2, 2^60 + 1] with the top one whole,
+// so they are the exact integers and no carry crosses more than one lane per
+// round; one lane resolves them at the end.  After the published bound of
+// (49 N + 57) / 17 divsteps (N = bits of m) g = 0 and f = +-gcd(x, m); the
+// kernel checks that and runs further batches if it ever were not so.
 // ---------------------------------------------------------------------------
-template <int WPL>
-struct WaveNum {
-  u32 w[WPL];
+constexpr int kSgBits = 60;
+constexpr u64 kSgMask = (1ull << kSgBits) - 1ull;
+constexpr int kSgLimbs = 128;  // 7680 bits: |values| stay below 2^(N + 10)
+
+struct SgNum {
+  i64 l[2];  // limbs 2 lane, 2 lane + 1
 };
 
-// a += b (SUB: a -= b) over the whole wave; returns the carry (borrow) out.
-template <int WPL, bool SUB>
-__device__ __forceinline__ bool wn_addsub(WaveNum<WPL>& a, const WaveNum<WPL>& b, u32 lane) {
-  bool gen, prop = true;
-  if constexpr (!SUB) {
-    u64 c = 0;
-#pragma unroll
-    for (int j = 0; j < WPL; ++j) {
-      u64 s = (u64)a.w[j] + b.w[j] + c;
-      a.w[j] = (u32)s;
-      c = s >> 32;
-      prop &= a.w[j] == 0xffffffffu;
-    }
-    gen = c != 0;
-  } else {
-    i64 br = 0;
-#pragma unroll
-    for (int j = 0; j < WPL; ++j) {
-      i64 s = (i64)a.w[j] - (i64)b.w[j] + br;
-      a.w[j] = (u32)s;
-      br = s >> 32;
-      prop &= a.w[j] == 0u;
-    }
-    gen = br != 0;
-  }
-  const u64 G = __builtin_amdgcn_ballot_w64(gen), P = __builtin_amdgcn_ballot_w64(prop), X = G | P;
-  const u64 S = X + G;
-  if (((S ^ X ^ G) >> lane) & 1u) {
-    bool go = true;
-#pragma unroll
-    for (int j = 0; j < WPL; ++j) {
-      if (go) {
-        if constexpr (!SUB) {
-          a.w[j] += 1u;
-          go = a.w[j] == 0u;
-        } else {
-          a.w[j] -= 1u;
-          go = a.w[j] == 0xffffffffu;
-        }
+__device__ __forceinline__ i64 sg_first64(i64 x) {  // lane 0's value, in every lane
+  const u32 lo = __builtin_amdgcn_readfirstlane((u32)(u64)x), hi = __builtin_amdgcn_readfirstlane((u32)((u64)x >> 32));
+  return (i64)(((u64)hi << 32) | lo);
+}
+
+// The low 64 bits of the exact integer (limb 0 and the low bits of limb 1).
+__device__ __forceinline__ u64 sg_low64(const SgNum& x) {
+  return (u64)sg_first64(x.l[0]) + ((u64)sg_first64(x.l[1]) << kSgBits);
+}
+
+// 60 divsteps (delta, f, g) -> (1 - delta, g, (g - f) / 2) if delta > 0 and g
+// is odd, else (1 + delta, f, (g + (g mod 2) f) / 2), on the low 64 bits, with
+// the matrix such that f' 2^60 = u f + v g and g' 2^60 = q f + r g.
+__device__ __forceinline__ int sg_divsteps(int delta, u64 f, u64 g, i64& u, i64& v, i64& q, i64& r) {
+  u = 1;
+  v = 0;
+  q = 0;
+  r = 1;
+  for (int i = 0; i < kSgBits; ++i) {
+    if (g & 1u) {
+      if (delta > 0) {
+        delta = 1 - delta;
+        const u64 t = f;
+        f = g;
+        g = g - t;
+        const i64 nu = 2 * q, nv = 2 * r;
+        q -= u;
+        r -= v;
+        u = nu;
+        v = nv;
+      } else {
+        delta = 1 + delta;
+        g = g + f;
+        q += u;
+        r += v;
+        u *= 2;
+        v *= 2;
       }
+    } else {
+      delta = 1 + delta;
+      u *= 2;
+      v *= 2;
     }
+    g = (u64)((i64)g >> 1);
   }
-  return S < X;
+  return delta;
 }
 
-// a >>= 1, with `top` shifted in as the new most significant bit.
-template <int WPL>
-__device__ __forceinline__ void wn_shr1(WaveNum<WPL>& a, u32 lane, u32 top) {
-  u32 nxt = (u32)__shfl_down((int)a.w[0], 1, 64);
-  if (lane == 63u) nxt = top;
+// out = (a x + b y + k m) / 2^60 (exact); out may be x or y.
+__device__ __forceinline__ void sg_lincomb(SgNum& out, i64 a, i64 b, const SgNum& x, const SgNum& y, u64 k,
+                                           const u64 (&ml)[2], u32 lane) {
+  __int128 P[2];
 #pragma unroll
-  for (int j = 0; j < WPL; ++j) a.w[j] = (a.w[j] >> 1) | ((j + 1 < WPL ? a.w[j + 1] : nxt) << 31);
+  for (int j = 0; j < 2; ++j)
+    P[j] = (__int128)a * x.l[j] + (__int128)b * y.l[j] + (__int128)((unsigned __int128)k * ml[j]);
+  const bool top = lane == 63u;
+  const i64 lo0 = (i64)((u64)P[0] & kSgMask), hi0 = (i64)(P[0] >> kSgBits);
+  const i64 lo1 = (i64)((u64)P[1] & kSgMask), hi1 = (i64)(P[1] >> kSgBits);
+  i64 below = __shfl_up(hi1, 1, 64);
+  if (lane == 0u) below = 0;
+  const i64 t0 = lo0 + below;
+  const i64 t1 = top ? (i64)P[1] + hi0 : lo1 + hi0;  // the top limb is never split
+  const i64 hb0 = t0 >> kSgBits, hb1 = t1 >> kSgBits;
+  i64 below2 = __shfl_up(hb1, 1, 64);
+  if (lane == 0u) below2 = 0;
+  const i64 s0 = (t0 & (i64)kSgMask) + below2;
+  const i64 s1 = top ? t1 + hb0 : (t1 & (i64)kSgMask) + hb0;
+  i64 up = __shfl_down(s0, 1, 64);  // limb 0 of the result is 0: shift down one limb
+  if (top) up = 0;
+  out.l[0] = s1;
+  out.l[1] = up;
 }
 
-template <int WPL>
-__device__ __forceinline__ bool wn_is_small(const WaveNum<WPL>& a, u32 lane, u32 value) {
-  bool other = false;
-#pragma unroll
-  for (int j = 0; j < WPL; ++j) other |= a.w[j] != ((lane == 0u && j == 0) ? value : 0u);
-  return __builtin_amdgcn_ballot_w64(other) == 0;
+// Limb `li` (60 bits) of a little-endian word array; words past nw are 0.
+__device__ __forceinline__ u64 sg_limb_of_words(const u32* w, int nw, int li) {
+  const int bit = li * kSgBits, w0 = bit >> 5, sh = bit & 31;
+  const u64 a = w0 < nw ? w[w0] : 0u, b = w0 + 1 < nw ? w[w0 + 1] : 0u, c = w0 + 2 < nw ? w[w0 + 2] : 0u;
+  u64 v = (a | (b << 32)) >> sh;
+  if (sh) v |= c << (64 - sh);
+  return v & kSgMask;
 }
 
-template <int WPL>
-__device__ __forceinline__ bool wn_even(const WaveNum<WPL>& a) {
-  return (__builtin_amdgcn_readfirstlane(a.w[0]) & 1u) == 0u;  // lane 0 holds the low word
-}
-
-// a/2 mod m for a < m, m odd
-template <int WPL>
-__device__ __forceinline__ void wn_halve_mod(WaveNum<WPL>& a, const WaveNum<WPL>& m, u32 lane) {
-  u32 top = 0;
-  if (!wn_even(a)) top = wn_addsub<WPL, false>(a, m, lane) ? 1u : 0u;
-  wn_shr1(a, lane, top);
-}
-
-// a - b mod m for a, b < m
-template <int WPL>
-__device__ __forceinline__ void wn_sub_mod(WaveNum<WPL>& a, const WaveNum<WPL>& b, const WaveNum<WPL>& m, u32 lane) {
-  if (wn_addsub<WPL, true>(a, b, lane)) wn_addsub<WPL, false>(a, m, lane);
+// One lane, in place: redundant limbs -> limbs in [0, 2^60); returns the carry
+// out (0 or -1: the sign of the exact integer).
+__device__ __forceinline__ i64 sg_resolve(i64* io) {
+  i64 c = 0;
+  for (int i = 0; i < kSgLimbs; ++i) {
+    const i64 v = io[i] + c;
+    io[i] = v & (i64)kSgMask;
+    c = v >> kSgBits;
+  }
+  return c;
 }
 
 // Inverse of one Montgomery-form value (R29), in place: a group of L lanes maps
-// it to canonical Z-space words, the whole wave runs the extended Euclid, the
-// group maps the result back (x R -> x^-1 R).  Sets kErrNotInvertible if
-// gcd(x, m) != 1.  One wave, launched once per batch.
+// it to canonical Z-space words, the whole wave runs the divsteps, one lane
+// resolves the result, and the group maps it back (x R -> x^-1 R).  Sets
+// kErrNotInvertible if gcd(x, m) != 1.  One wave, launched once per batch.
 template <int L, int K, int W>
 __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __restrict__ m_words, u32* xio, u32* err) {
   constexpr int NL = L * K;
-  constexpr int WPL = W / 64;
-  static_assert(W % 64 == 0, "wave-wide numbers need W divisible by 64");
+  static_assert(kSgBits * kSgLimbs >= 32 * W + 64, "room for the divsteps numbers");
+  static_assert(29 * NL >= 32 * (W + 1), "room for the result before reduction");
   __shared__ u32 lds[NL + 2 * Cfg<L, K, W>::S];
-  __shared__ u32 words[W];
+  __shared__ u32 words[W + 2];
+  __shared__ i64 res[3][kSgLimbs];
+  __shared__ int s_state;  // 0 running, 1 done (+2: negate), -1 not invertible
   for (int i = threadIdx.x; i < NL; i += 64) lds[i] = mod.m[i];
   __syncthreads();
   Grp<L, K, W> g;
@@ -1205,52 +1230,110 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
   }
   __syncthreads();
   const u32 lane = threadIdx.x;
-  WaveNum<WPL> u, v, x1, x2, m;
+  SgNum f, gg, d, e;
+  u64 ml[2];
 #pragma unroll
-  for (int j = 0; j < WPL; ++j) {
-    u.w[j] = words[lane * WPL + j];
-    m.w[j] = m_words[lane * WPL + j];
-    v.w[j] = m.w[j];
-    x1.w[j] = (lane == 0u && j == 0) ? 1u : 0u;
-    x2.w[j] = 0u;
+  for (int j = 0; j < 2; ++j) {
+    ml[j] = sg_limb_of_words(m_words, W, (int)(2 * lane + j));
+    f.l[j] = (i64)ml[j];
+    gg.l[j] = (i64)sg_limb_of_words(words, W, (int)(2 * lane + j));
+    d.l[j] = 0;
+    e.l[j] = (lane == 0u && j == 0) ? 1 : 0;
   }
-  // invariant: x1 * x = u, x2 * x = v (mod m)
-  bool fail = wn_is_small(u, lane, 0u);
-  const int max_steps = 4 * 32 * W + 64;
-  int steps = 0;
-  while (!fail && !wn_is_small(u, lane, 1u) && !wn_is_small(v, lane, 1u)) {
-    while (wn_even(u)) {
-      wn_shr1(u, lane, 0u);
-      wn_halve_mod(x1, m, lane);
+  // m^-1 mod 2^64 by Newton's iteration (m odd: m m = 1 mod 8 to start)
+  const u64 m0 = (u64)m_words[0] | ((u64)m_words[1] << 32);
+  u64 minv = m0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) minv *= 2u - m0 * minv;
+  const int nbits = 32 * W;
+  const int batches = ((49 * nbits + 57) / 17 + kSgBits) / kSgBits;
+  int delta = 1, done = 0;
+  for (;;) {
+    const int todo = done == 0 ? batches : 8;
+    for (int bi = 0; bi < todo; ++bi) {
+      i64 u, v, q, r;
+      delta = sg_divsteps(delta, sg_low64(f), sg_low64(gg), u, v, q, r);
+      const u64 d0 = (u64)sg_first64(d.l[0]), e0 = (u64)sg_first64(e.l[0]);
+      const u64 kd = (0u - ((u64)u * d0 + (u64)v * e0) * minv) & kSgMask;
+      const u64 ke = (0u - ((u64)q * d0 + (u64)r * e0) * minv) & kSgMask;
+      const u64 none[2] = {0u, 0u};
+      SgNum nf, ng, nd, ne;
+      sg_lincomb(nf, u, v, f, gg, 0u, none, lane);
+      sg_lincomb(ng, q, r, f, gg, 0u, none, lane);
+      sg_lincomb(nd, u, v, d, e, kd, ml, lane);
+      sg_lincomb(ne, q, r, d, e, ke, ml, lane);
+      f = nf;
+      gg = ng;
+      d = nd;
+      e = ne;
     }
-    while (wn_even(v)) {
-      wn_shr1(v, lane, 0u);
-      wn_halve_mod(x2, m, lane);
+    done += todo;
+    // resolve f, g, d in lane 0
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      res[0][2 * lane + j] = f.l[j];
+      res[1][2 * lane + j] = gg.l[j];
+      res[2][2 * lane + j] = d.l[j];
     }
-    WaveNum<WPL> d = u;
-    if (!wn_addsub<WPL, true>(d, v, lane)) {  // u >= v
-      u = d;
-      wn_sub_mod(x1, x2, m, lane);
-    } else {
-      wn_addsub<WPL, true>(v, u, lane);
-      wn_sub_mod(x2, x1, m, lane);
+    __syncthreads();
+    if (lane == 0u) {
+      i64 *rf = res[0], *rg = res[1], *rd = res[2];
+      const i64 cf = sg_resolve(rf), cg = sg_resolve(rg);
+      bool g_zero = cg == 0 && rg[0] == 0, f_one = cf == 0 && rf[0] == 1;
+      bool f_mone = cf == -1 && rf[0] == (i64)kSgMask;
+      for (int i = 1; i < kSgLimbs; ++i) {
+        g_zero &= rg[i] == 0;
+        f_one &= rf[i] == 0;
+        f_mone &= rf[i] == (i64)kSgMask;
+      }
+      if (g_zero && (f_one || f_mone)) {
+        // |d| as W + 1 words; the sign of d f into the state
+        const bool dneg = sg_resolve(rd) < 0;
+        if (dneg) {
+          i64 c = 1;
+          for (int i = 0; i < kSgLimbs; ++i) {
+            const i64 t = ((~rd[i]) & (i64)kSgMask) + c;
+            rd[i] = t & (i64)kSgMask;
+            c = t >> kSgBits;
+          }
+        }
+        for (int w = 0; w < W + 1; ++w) {
+          const int bit = 32 * w, li = bit / kSgBits, sh = bit - kSgBits * li;
+          u64 v = (u64)rd[li] >> sh;
+          if (li + 1 < kSgLimbs && sh > kSgBits - 32) v |= (u64)rd[li + 1] << (kSgBits - sh);
+          words[w] = (u32)v;
+        }
+        s_state = 1 + ((dneg != f_mone) ? 2 : 0);
+      } else if (g_zero) {
+        s_state = -1;  // f = +-gcd(x, m) != +-1
+      } else {
+        s_state = done >= 4 * batches ? -1 : 0;
+      }
     }
-    if (wn_is_small(u, lane, 0u) || wn_is_small(v, lane, 0u) || ++steps > max_steps) fail = true;
+    __syncthreads();
+    if (s_state != 0) break;  // uniform
   }
-  if (fail) {
+  if (s_state < 0) {
     if (lane == 0u) atomicOr(err, kErrNotInvertible);
-    return;  // uniform: every lane saw the same ballots
+    return;  // uniform
   }
-  const bool u_one = wn_is_small(u, lane, 1u);
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < WPL; ++j) words[lane * WPL + j] = u_one ? x1.w[j] : x2.w[j];
+  const bool neg = (s_state & 2) != 0;
   __syncthreads();
   if (in_grp) {
     u32 x[K];
-    words_to_limbs<L, K>(words, W, 0, x);
+    words_to_limbs<L, K>(words, W + 1, 0, x);
     g.set(g.A, x);
-    g.mul_global(mod.r2);
+    g.mul_global(mod.r2);  // |d| R mod m (|d| < R)
+    if (neg) {
+      get_lds<L, K>(x, g.A, g.lid);
+      canonicalize<L, K>(x, g.md);
+      i64 t[K];
+#pragma unroll
+      for (int i = 0; i < K; ++i) t[i] = (i64)g.md.m[i * L + g.lid] - (i64)x[i];
+      u32 y[K];
+      norm_full<L, K>(t, y);  // m - |d| R: in [1, m) since the value is invertible
+      g.set(g.A, y);
+    }
     g.save(xio, g.A);
   }
 }
