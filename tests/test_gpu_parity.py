@@ -154,6 +154,39 @@ def test_negate_sub(eng, which, request):
 
 
 @pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_inverse_edge_values(eng, which, request):
+    # Negate inverts mod n^2 (evaluator.cc:90-93): edge values through the
+    # single-inversion kernel (one element: its own batch) and as one batch,
+    # against the oracle (OpenSSL BN_mod_inverse) and the closed form; then a
+    # value sharing the factor p with n^2 must be reported as not invertible
+    kd, gk, ok = request.getfixturevalue(which)
+    n2 = kd["n"] ** 2
+    W = gk.ct_words
+    rng = np.random.default_rng(31)
+    vals = [1, 2, 3, n2 - 1, n2 - 2, 1 << (n2.bit_length() - 2), (n2 - 1) // 2,
+            int(rng.integers(1, 2**62)), int.from_bytes(rng.bytes(8 * W), "little") % n2]
+    rows = np.stack([eng.int_to_words(v, W) for v in vals])
+    want = ok.negate(rows)
+    assert [eng.words_to_int(r) for r in want] == [pow(v, -1, n2) for v in vals]
+    for i, v in enumerate(vals):
+        print(f"[{which}] inverse of edge value {i}", flush=True)
+        got = gk.negate(rows[i : i + 1])
+        gk.ctx.sync()
+        assert np.array_equal(host(got)[0], want[i]), i
+    print(f"[{which}] inverse of all edge values", flush=True)
+    got = gk.negate(rows)
+    gk.ctx.sync()
+    assert np.array_equal(host(got), want)
+    from heu_amd._native import HeuError
+
+    print(f"[{which}] inverse of a multiple of p", flush=True)
+    with pytest.raises(HeuError):
+        gk.negate(np.stack([eng.int_to_words(kd["p"] * 12345, W)]))
+        gk.ctx.sync()
+    gk.ctx.sync()  # the error was reported once and cleared
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
 def test_sum_dot_segments(eng, which, request):
     kd, gk, ok = request.getfixturevalue(which)
     n = 150
