@@ -208,7 +208,8 @@ struct Job {
   size_t in_elem[2] = {0, 0};
   void* out = nullptr;
   size_t out_elem = 0;
-  bool done = false;
+  bool ran = false;   // set by the thread that ran it
+  bool done = false;  // set under the queue lock once its results are final
   std::exception_ptr err;
 };
 
@@ -367,6 +368,7 @@ void run_group(const std::vector<Job*>& g) {
       off += j->count * j0.out_elem;
     }
     call.finish();
+    for (Job* j : g) j->ran = true;
     Runtime& rt = Runtime::get();
     std::lock_guard<std::mutex> lk(rt.qmu);
     rt.stats.device_calls += 1;
@@ -374,6 +376,7 @@ void run_group(const std::vector<Job*>& g) {
   } catch (...) {
     if (g.size() == 1) {
       g[0]->err = std::current_exception();
+      g[0]->ran = true;
       return;
     }
     for (Job* j : g) run_group(std::vector<Job*>{j});
@@ -392,15 +395,21 @@ void submit(Job& j) {
       std::vector<Job*> take;
       take.swap(rt.pending);
       lk.unlock();
-      std::map<std::tuple<int, heu_key*, uint32_t>, std::vector<Job*>> groups;
-      std::vector<std::tuple<int, heu_key*, uint32_t>> order;
-      for (Job* t : take) {
-        auto k = std::make_tuple(t->op, t->key, t->param);
-        auto& v = groups[k];
-        if (v.empty()) order.push_back(k);
-        v.push_back(t);
+      try {
+        std::map<std::tuple<int, heu_key*, uint32_t>, std::vector<Job*>> groups;
+        std::vector<std::tuple<int, heu_key*, uint32_t>> order;
+        for (Job* t : take) {
+          auto k = std::make_tuple(t->op, t->key, t->param);
+          auto& v = groups[k];
+          if (v.empty()) order.push_back(k);
+          v.push_back(t);
+        }
+        for (auto& k : order) run_group(groups[k]);
+      } catch (...) {
+        // run_group keeps errors per call; anything else fails the calls not yet run
+        for (Job* t : take)
+          if (!t->ran) t->err = std::current_exception();
       }
-      for (auto& k : order) run_group(groups[k]);
       lk.lock();
       for (Job* t : take) t->done = true;
       rt.running = false;
