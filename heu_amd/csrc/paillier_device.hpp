@@ -1115,38 +1115,43 @@ __device__ __forceinline__ u64 sg_low64(const SgNum& x) {
 
 // 60 divsteps (delta, f, g) -> (1 - delta, g, (g - f) / 2) if delta > 0 and g
 // is odd, else (1 + delta, f, (g + (g mod 2) f) / 2), on the low 64 bits, with
-// the matrix such that f' 2^60 = u f + v g and g' 2^60 = q f + r g.
+// the matrix such that f' 2^60 = u f + v g and g' 2^60 = q f + r g.  A run of
+// z even steps is taken at once (z = trailing zeros of g): g >>= z, u, v <<= z.
 __device__ __forceinline__ int sg_divsteps(int delta, u64 f, u64 g, i64& u, i64& v, i64& q, i64& r) {
   u = 1;
   v = 0;
   q = 0;
   r = 1;
-  for (int i = 0; i < kSgBits; ++i) {
-    if (g & 1u) {
-      if (delta > 0) {
-        delta = 1 - delta;
-        const u64 t = f;
-        f = g;
-        g = g - t;
-        const i64 nu = 2 * q, nv = 2 * r;
-        q -= u;
-        r -= v;
-        u = nu;
-        v = nv;
-      } else {
-        delta = 1 + delta;
-        g = g + f;
-        q += u;
-        r += v;
-        u *= 2;
-        v *= 2;
-      }
+  int i = kSgBits;
+  while (i > 0) {
+    int z = g ? __builtin_ctzll(g) : i;
+    z = z < i ? z : i;
+    g = (u64)((i64)g >> z);
+    u = (i64)((u64)u << z);
+    v = (i64)((u64)v << z);
+    delta += z;
+    i -= z;
+    if (i == 0) break;
+    if (delta > 0) {  // g odd
+      delta = 1 - delta;
+      const u64 t = f;
+      f = g;
+      g = g - t;
+      const i64 nu = 2 * q, nv = 2 * r;
+      q -= u;
+      r -= v;
+      u = nu;
+      v = nv;
     } else {
       delta = 1 + delta;
+      g = g + f;
+      q += u;
+      r += v;
       u *= 2;
       v *= 2;
     }
     g = (u64)((i64)g >> 1);
+    --i;
   }
   return delta;
 }
