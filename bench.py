@@ -306,6 +306,12 @@ def main():
             "avg_launch_ms": dom_ms / dom_launches,
             "launches": dom_launches,
             "algorithmic_macs_per_element": ALGO_MACS[dom],
+            # integer multiply-add work, neither HBM- nor MFMA-bound: the peak is the
+            # VALU's, which equals MI355X's published FP64 vector rate (78.6 TFLOP/s =
+            # 39.3 T FMA/s), since v_mad_u64_u32 issues at the v_fma_f64 rate
+            "peak_source": "MI355X FP64 vector peak 78.6 TFLOP/s = 39.3 T FMA/s (64 lanes x 256 CUs x 2.4 GHz); "
+                           "v_mad_u64_u32 issues at the v_fma_f64 rate (profiles/valu_issue_rates_probe.md)",
+            "bytes_per_launch_algorithmic": ALGO_BYTES[dom] * elems_per_launch,
         },
         "kernel_ms": {n: kern[n][0] for n in kern},
         "kernels": kernels,
