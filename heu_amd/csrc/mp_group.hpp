@@ -18,9 +18,9 @@
 // for K <= 31 no carry step is needed at all.  K = 36 multiplies take a
 // one-step carry over all slots every 31 iterations; K = 36 squarings (whose
 // bound is 27 iterations, below) carry one slot per iteration instead, the one
-// in the middle of the lane, which hands its bits above 29 to the slot above it
-// and so cuts every slot's life into runs of <= K/2 + 1 iterations between
-// carries.  With R = 2^(29*NL) >= 4m, inputs below 2m give
+// in the middle of the lane, which hands its bits above 32 to the slot above it
+// (one multiply-add by 2^3) and so cuts every slot's life into runs of
+// <= K/2 + 1 iterations between carries.  With R = 2^(29*NL) >= 4m, inputs below 2m give
 // outputs below 2m, so chained multiplications never need the conditional
 // subtraction; only results leaving the engine are canonicalised.
 //
@@ -118,6 +118,21 @@ __device__ __forceinline__ u32 grp_shl1_z(u32 x, u32 lid) {
   } else {
     return grp_shl1<L>(x, lid);
   }
+}
+
+// Groups within a quad (L <= 4): broadcast of group lane 0 and the rotation of
+// grp_shl1_z as DPP moves with bound_ctrl set, which changes nothing here
+// (every lane reads an active lane of its own quad) but lets an AND applied to
+// the result fuse with the move into one v_and_b32 with a DPP operand.
+template <int L>
+__device__ __forceinline__ u32 quad_bcast0(u32 x) {
+  static_assert(L == 2 || L == 4, "groups within a quad");
+  return __builtin_amdgcn_mov_dpp(x, L == 2 ? quad_perm(0, 0, 2, 2) : quad_perm(0, 0, 0, 0), 0xf, 0xf, true);
+}
+template <int L>
+__device__ __forceinline__ u32 quad_rot1(u32 x) {
+  static_assert(L == 2 || L == 4, "groups within a quad");
+  return __builtin_amdgcn_mov_dpp(x, L == 2 ? quad_perm(1, 0, 3, 2) : quad_perm(1, 2, 3, 0), 0xf, 0xf, true);
 }
 
 template <int L>
@@ -251,6 +266,13 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
   // multiply spills registers).
   constexpr int kMid = K / 2;
   static_assert(K <= kNorm || (K - 1 - kMid <= kNorm && kMid + 1 <= kNorm), "carry runs too long");
+  // 2^(32-29) for the squaring's carry, hidden from the optimizer, which would
+  // otherwise turn the one multiply-add into a 64-bit shift, two masks and an add.
+  u32 hi_weight = 1u << (32 - kLimbBits);
+  if constexpr (K > kNorm && kSqr) asm volatile("" : "+s"(hi_weight));
+  // The limb mask in a register: a DPP operand's partner cannot be a literal.
+  u32 mask_v = kLimbMask;
+  if constexpr (L <= 4) asm volatile("" : "+v"(mask_v));
 #pragma unroll
   for (int t = 0; t < K; ++t) acc[t] = 0;
   const u32 lid = grp_lane<L>();
@@ -275,18 +297,28 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
 #pragma unroll
         for (int j = 0; j < K; ++j) acc[(j + s) % K] = mad64(ai, b[j], acc[(j + s) % K]);
       }
-      u32 q = ((u32)acc[s % K] * md.mprime) & kLimbMask;
-      q = grp_bcast0<L>(q);
+      u32 q;
+      if constexpr (L <= 4)
+        q = quad_bcast0<L>((u32)acc[s % K] * md.mprime) & mask_v;
+      else
+        q = grp_bcast0<L>(((u32)acc[s % K] * md.mprime) & kLimbMask);
 #pragma unroll
       for (int j = 0; j < K; ++j) acc[(j + s) % K] = mad64(q, m[j], acc[(j + s) % K]);
       const u64 out0 = acc[s % K];
       acc[(s + 1) % K] += out0 >> kLimbBits;
-      acc[s % K] = (u64)grp_shl1_z<L>((u32)out0 & kLimbMask, lid);  // group lane 0 sends 0
+      // group lane 0 sends 0 (its low 29 bits are 0; bits 29-31 are masked off)
+      if constexpr (L <= 4)
+        acc[s % K] = (u64)(quad_rot1<L>((u32)out0) & mask_v);
+      else
+        acc[s % K] = (u64)grp_shl1_z<L>((u32)out0 & kLimbMask, lid);
       if constexpr (K > kNorm && kSqr) {
-        // the slot that is logical kMid in the next iteration
+        // The slot that is logical kMid in the next iteration, split at bit 32:
+        // its high word is worth 2^3 units of the slot above (one multiply-add).
+        // The low word left behind (< 2^32) is as harmless as a 29-bit limb.
         u64& x = acc[(kMid + s + 1) % K];
-        acc[(kMid + 1 + s + 1) % K] += x >> kLimbBits;
-        x &= kLimbMask;
+        u64& above = acc[(kMid + 1 + s + 1) % K];
+        above = mad64((u32)(x >> 32), hi_weight, above);
+        x = (u32)x;
       } else if constexpr (K > kNorm && ((s + 1) % kNorm == 0 || s == K - 1)) {
         norm_step<L, K, (s + 1) % K>(acc, lid);
       }
