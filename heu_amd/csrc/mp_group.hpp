@@ -255,21 +255,23 @@ constexpr bool sq_cross_here(int x, int y) {
   return (d >= 1 && d < K / 2) || (d == K / 2 && x < K / 2);
 }
 
-template <int L, int K, bool kSqr>
+template <int L, int K, bool kSqr, bool kSlotMul = false>
 __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
                                           const GMod<L, K>& md) {
   constexpr int kNorm = kSqr ? kNormEverySq : kNormEvery;
   // Carries when K > kNorm.  Squaring: the middle slot each iteration (runs of
   // K-1-kMid and kMid+1 iterations).  Multiplication: a carry step over all
-  // slots every kNorm iterations and at the end of each block (the per-
-  // iteration form makes the compiler interleave iterations until the K = 36
-  // multiply spills registers).
+  // slots every kNorm iterations and at the end of each block, or (kSlotMul)
+  // the squaring's per-iteration middle-slot carry.  The per-iteration form
+  // issues fewer instructions, but lets the compiler interleave iterations
+  // further, which spills registers in kernels that also square (K = 36).
   constexpr int kMid = K / 2;
   static_assert(K <= kNorm || (K - 1 - kMid <= kNorm && kMid + 1 <= kNorm), "carry runs too long");
   // 2^(32-29) for the squaring's carry, hidden from the optimizer, which would
   // otherwise turn the one multiply-add into a 64-bit shift, two masks and an add.
   u32 hi_weight = 1u << (32 - kLimbBits);
-  if constexpr (K > kNorm && kSqr) asm volatile("" : "+s"(hi_weight));
+  constexpr bool kSlot = K > kNorm && (kSqr || kSlotMul);
+  if constexpr (kSlot) asm volatile("" : "+s"(hi_weight));
   // The limb mask in a register: a DPP operand's partner cannot be a literal.
   u32 mask_v = kLimbMask;
   if constexpr (L <= 4) asm volatile("" : "+v"(mask_v));
@@ -311,7 +313,7 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
         acc[s % K] = (u64)(quad_rot1<L>((u32)out0) & mask_v);
       else
         acc[s % K] = (u64)grp_shl1_z<L>((u32)out0 & kLimbMask, lid);
-      if constexpr (K > kNorm && kSqr) {
+      if constexpr (kSlot) {
         // The slot that is logical kMid in the next iteration, split at bit 32:
         // its high word is worth 2^3 units of the slot above (one multiply-add).
         // The low word left behind (< 2^32) is as harmless as a 29-bit limb.
@@ -326,10 +328,10 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
   }
 }
 
-template <int L, int K>
+template <int L, int K, bool kSlotMul = false>
 __device__ __forceinline__ void mont_mul(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
                                          const GMod<L, K>& md) {
-  mont_core<L, K, false>(acc, a_lds, b, md);
+  mont_core<L, K, false, kSlotMul>(acc, a_lds, b, md);
 }
 
 // acc = a * a * R^-1 (mod m), lazy; b holds this lane's limbs of a_lds.

@@ -48,10 +48,11 @@ struct Grp {
   GMod<L, K> md;
   u32 lid;
 
-  // A <- A * b * R^-1
+  // A <- A * b * R^-1 (kSlotMul: the carry form for multiply-only kernels)
+  template <bool kSlotMul = false>
   __device__ __forceinline__ void mul(const u32 (&b)[K]) {
     u64 acc[K];
-    mont_mul<L, K>(acc, A, b, md);
+    mont_mul<L, K, kSlotMul>(acc, A, b, md);
     u32 x[K];
     seminorm<L, K>(acc, x);
     put_lds<L, K>(A, x, lid);
@@ -62,10 +63,11 @@ struct Grp {
     mont_mul<L, K>(acc, slot, b, md);
     seminorm<L, K>(acc, out);
   }
+  template <bool kSlotMul = false>
   __device__ __forceinline__ void mul_global(const u32* __restrict__ src) {
     u32 b[K];
     load_r29<L, K>(b, src, lid);
-    mul(b);
+    mul<kSlotMul>(b);
   }
   __device__ __forceinline__ void square() {
     u32 b[K];
@@ -221,7 +223,7 @@ __device__ __forceinline__ void fixed_base_pow(Grp<L, K, W>& g, const PubArgs& P
   const long per = 1l << P.unit;
   g.set_global(g.A, P.table + (long)r_digit(rp, r_words, 0, P.unit) * NL);
   for (int w = 1; w < P.windows; ++w)
-    g.mul_global(P.table + (w * per + r_digit(rp, r_words, w * P.unit, P.unit)) * NL);
+    g.template mul_global<true>(P.table + (w * per + r_digit(rp, r_words, w * P.unit, P.unit)) * NL);
 }
 
 // ---------------------------------------------------------------------------
