@@ -111,7 +111,7 @@ def check(rc: int) -> None:
 
 class HeuServerConfig(ctypes.Structure):
     _fields_ = [("workspace_bytes", SZ), ("max_inflight", U32), ("max_batch_tasks", U32),
-                ("batch_rounds", U32), ("parts_per_round", U32)]
+                ("batch_rounds", U32), ("parts_per_round", U32), ("urgent_ms", U32)]
 
 
 class HeuServerStats(ctypes.Structure):

@@ -330,12 +330,14 @@ def _summary(rate, lat, wall):
         "requests": len(lat),
         "throughput_req_per_s": len(lat) / wall,
         "p50_ms": float(np.percentile(lat, 50)),
+        "p95_ms": float(np.percentile(lat, 95)),
         "p99_ms": float(np.percentile(lat, 99)),
+        "max_ms": float(np.max(lat)),
         "mean_ms": float(np.mean(lat)),
     }
 
 
-def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0), per_rate: int = 160) -> dict:
+def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0), per_rate: int = 320) -> dict:
     pools = []
     for i in range(32):
         bits = 2048 if i < 24 else 3072

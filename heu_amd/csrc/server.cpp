@@ -7,14 +7,17 @@
 // scheduler retires finished batches (stream events), then, while a stream
 // slot is free, builds the next batch:
 //   - order the active requests by remaining work, shortest first, aged (the
-//     key shrinks the longer a request waits, so long ones are not starved);
+//     key shrinks the longer a request waits); requests that have waited
+//     longer than urgent_ms go before all others, oldest first, so a long
+//     request is not held back for good by a stream of short ones;
 //   - the batch takes the kernel shape of the first request whose shape has
 //     room (each shape may hold batch_rounds rounds in flight across all slots,
 //     decryption only a share of that, which keeps CUs free for the short ops
 //     that arrive during a decryption's long units);
 //   - round-robin over the requests of that shape, each pass carving the next
 //     contiguous range of up to a quantum (1/parts_per_round of a round) from
-//     every request, until the room is used (a sum or dot is taken whole);
+//     every request, until the room is used (a sum or dot is taken whole); an
+//     urgent request at the head of the order takes all the room it can use;
 //   - trim to the longest prefix whose plan fits the largest free block of the
 //     workspace arena, plan it into that block, copy its work lists up from
 //     pinned staging and enqueue it on the slot's stream.
@@ -235,12 +238,16 @@ bool heu_server::retire() {
 // Picks the parts of the next batch (called with mu held).  False if no
 // request's shape has room.
 bool heu_server::select(std::vector<Pick>* picks, const heu_hip::WorklistShape** shape_out) {
+  // key: remaining rounds, aged; urgent requests sort first by their (negated) age
   std::vector<std::pair<double, uint64_t>> order;
   const auto now = Clock::now();
   for (uint64_t t : pending) {
     const Request& r = reqs[t];
     const double age_ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
-    order.emplace_back((double)r.remaining_units() / (double)r.round / (1.0 + age_ms / kAgingMs), t);
+    if (age_ms > (double)cfg.urgent_ms)
+      order.emplace_back(-1.0 - age_ms, t);
+    else
+      order.emplace_back((double)r.remaining_units() / (double)r.round / (1.0 + age_ms / kAgingMs), t);
   }
   std::stable_sort(order.begin(), order.end(),
                    [](const std::pair<double, uint64_t>& x, const std::pair<double, uint64_t>& y) {
@@ -281,6 +288,7 @@ bool heu_server::select(std::vector<Pick>* picks, const heu_hip::WorklistShape**
       if (r.shape != shape) continue;
       auto at = mine.find(t);
       const bool first = picks->empty() && idle;
+      const bool urgent = picks->empty() && ot.first < 0.0;
       if (r.whole) {
         if (r.admitted || at != mine.end()) continue;
         if (!first && used + r.whole_units > room) continue;
@@ -292,7 +300,9 @@ bool heu_server::select(std::vector<Pick>* picks, const heu_hip::WorklistShape**
         const long have = at == mine.end() ? 0 : (*picks)[at->second].count;
         const long rem = r.n - r.admitted - have;
         if (rem <= 0) continue;
-        const long avail = first ? quantum : std::min(quantum, room - used);
+        const long avail = urgent  ? (first ? std::max(quantum, room - used) : room - used)
+                           : first ? quantum
+                                   : std::min(quantum, room - used);
         const long elems = std::min(rem, avail / r.factor * r.gpu);
         if (elems <= 0) continue;
         const long before = r.units_for(have);
@@ -459,6 +469,7 @@ int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** o
   if (!srv->cfg.max_batch_tasks) srv->cfg.max_batch_tasks = 512;
   if (!srv->cfg.batch_rounds) srv->cfg.batch_rounds = 1;
   if (!srv->cfg.parts_per_round) srv->cfg.parts_per_round = 8;
+  if (!srv->cfg.urgent_ms) srv->cfg.urgent_ms = 300;
   if (srv->cfg.max_inflight > 8) return fail(HEU_EINVAL, "max_inflight must be at most 8");
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = srv->ws.alloc(round256(srv->cfg.workspace_bytes));

@@ -243,7 +243,9 @@ int heu_ctx_set_grid_multiplier(heu_ctx *ctx, uint32_t key_bits, int decrypt_sha
  * at once), and every batch takes parts round-robin across the active requests,
  * shortest remaining work first, so
  * a long request shares the device with the short ones that arrive during it
- * instead of blocking them (sums and dots stay whole).  A request completes
+ * instead of blocking them (sums and dots stay whole).  A request that has
+ * waited longer than urgent_ms goes first, which bounds how long a long
+ * request can be held back by a stream of short ones.  A request completes
  * when its last part does, and its result is independent of what it was
  * batched with.  HEU has no counterpart; it serves the request-loop shape of
  * the path (keys and tables resident, requests of mixed sizes, keys and ops). */
@@ -251,11 +253,13 @@ typedef struct heu_server heu_server;
 
 typedef struct heu_server_config {
   size_t workspace_bytes;   /* device workspace for in-flight batches (0: 8 GiB) */
-  uint32_t max_inflight;    /* batches on the device at once (0: 2) */
+  uint32_t max_inflight;    /* batches on the device at once (0: 4, at most 8) */
   uint32_t max_batch_tasks; /* request parts per batch (0: 512) */
   uint32_t batch_rounds;    /* work-list units per batch and kernel shape, in rounds of the
                                units the device runs at once (0: 1) */
-  uint32_t parts_per_round; /* elementwise requests are cut into parts of 1/parts_per_round of a round (0: 4) */
+  uint32_t parts_per_round; /* elementwise requests are cut into parts of 1/parts_per_round of a round (0: 8) */
+  uint32_t urgent_ms;       /* a request waiting longer than this goes ahead of every newer one and takes
+                               its shape's whole room, oldest first (0: 300; UINT32_MAX: never) */
 } heu_server_config;
 
 typedef struct heu_server_counters {

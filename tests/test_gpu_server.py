@@ -111,6 +111,25 @@ def test_alone_equals_mix(eng, ctx, keys):
     srv.close()
 
 
+def test_urgent_requests_stay_exact(eng, ctx, keys):
+    # urgent_ms = 1: almost every request is urgent by the time it is picked, so
+    # batches take the oldest request's whole room first; results stay exact
+    reqs = make_requests(eng, keys, 24, 8)
+    ctx.sync()
+    srv = eng.Server(ctx, workspace_bytes=4 << 30, max_inflight=2, urgent_ms=1)
+    print("server (urgent_ms=1): submitting 24 requests", flush=True)
+    pending = [(op, key, args, *submit(srv, key, op, args)) for op, key, args in reqs]
+    for _, _, _, t, _ in pending:
+        srv.wait(t, 60000.0)
+    st = srv.stats()
+    assert st["completed"] == 24 and st["failed"] == 0
+    for op, key, args, _, out in pending:
+        want = alone(key, op, args)
+        ctx.sync()
+        assert np.array_equal(host(out), host(want)), op
+    srv.close()
+
+
 def test_server_admission_limits(eng, ctx, keys):
     # a workspace too small for even one part of a big decryption fails that
     # request and keeps serving others
