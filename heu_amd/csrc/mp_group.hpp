@@ -135,6 +135,15 @@ __device__ __forceinline__ u32 quad_rot1(u32 x) {
   return __builtin_amdgcn_mov_dpp(x, L == 2 ? quad_perm(1, 0, 3, 2) : quad_perm(1, 2, 3, 0), 0xf, 0xf, true);
 }
 
+// Wider groups (L >= 8): lane i receives lane i+1's value AND mask, the top
+// lane of the group 0.  The DPP shift (bound_ctrl: lane 15 of a row reads 0)
+// fuses with the AND; the top lane of a group inside the row is cleared by a
+// select, as the next group's lane is not read (groups may diverge).
+__device__ __forceinline__ u32 row_shl1_masked(u32 x, u32 mask, bool top) {
+  const u32 v = __builtin_amdgcn_mov_dpp(x, kDppRowShl1, 0xf, 0xf, true) & mask;
+  return top ? 0u : v;
+}
+
 template <int L>
 __device__ __forceinline__ u64 grp_shr1_64(u64 x, u32 lid) {
   u32 lo = grp_shr1<L>((u32)x, lid), hi = grp_shr1<L>((u32)(x >> 32), lid);
@@ -274,7 +283,7 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
   if constexpr (kSlot) asm volatile("" : "+s"(hi_weight));
   // The limb mask in a register: a DPP operand's partner cannot be a literal.
   u32 mask_v = kLimbMask;
-  if constexpr (L <= 4) asm volatile("" : "+v"(mask_v));
+  asm volatile("" : "+v"(mask_v));
 #pragma unroll
   for (int t = 0; t < K; ++t) acc[t] = 0;
   const u32 lid = grp_lane<L>();
@@ -312,7 +321,7 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
       if constexpr (L <= 4)
         acc[s % K] = (u64)(quad_rot1<L>((u32)out0) & mask_v);
       else
-        acc[s % K] = (u64)grp_shl1_z<L>((u32)out0 & kLimbMask, lid);
+        acc[s % K] = (u64)(row_shl1_masked((u32)out0, mask_v, lid == (u32)(L - 1)));
       if constexpr (kSlot) {
         // The slot that is logical kMid in the next iteration, split at bit 32:
         // its high word is worth 2^3 units of the slot above (one multiply-add).
