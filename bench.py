@@ -133,8 +133,9 @@ def cpu_baseline(kd, m, r, k):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this box (every run here "
-              "uses a single MI355X; see DESIGN.md for how the path shards).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path runs on a single "
+              "MI355X (sharding by element ranges, one process per GPU, is designed but not built).",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (paillier_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg.  The product package
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its result and by bench.py's cpu_baseline leg.  The product package
 (heu_amd) never imports this module.
 
 Big integers travel as numpy uint32 arrays of little-endian limbs; signed

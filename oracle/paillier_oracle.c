@@ -2,7 +2,7 @@
  * paillier_oracle.c -- CPU restatement of HEU's paillier_zahlen (ZPaillier)
  * over OpenSSL 3 BIGNUM.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
+ * TEST INFRASTRUCTURE ONLY.  It checks the engine's results: only tests/, the
  * smoke() check of __graft_entry__.py and bench.py's cpu_baseline leg may
  * load it.  The product library (heu_amd/) never links or calls it.
  *

@@ -1,5 +1,5 @@
-"""Pins the CPU oracle (oracle/paillier_oracle.c) before it is trusted as the
-checker: against the golden vectors written from Python closed forms, against
+"""Pins the CPU oracle (oracle/paillier_oracle.c) before it is trusted to
+check results: against the golden vectors written from Python closed forms, against
 brute force on a toy key, and against the invariants of the reference's own
 tests (paillier_zahlen/key_generator_test.cc:21-71, paillier_test.cc:40-233,
 phe/encryptor_test.cc:64-86)."""
