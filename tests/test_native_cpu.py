@@ -1,6 +1,7 @@
 """CPU-side checks of the engine library: it loads, exports every entry point
-declared in include/heu_paillier.h, and its host logic (seeded key generation,
-the counter PRNG, limb conversions) agrees with the oracle.  No kernel runs."""
+declared in include/heu_paillier.h, the ctypes structures lay out like the
+header's, and its host logic (seeded key generation, the counter PRNG, limb
+conversions) agrees with the oracle.  No kernel runs."""
 import math
 import os
 import re
@@ -33,6 +34,46 @@ def test_library_exports_every_declared_symbol(native):
     for s in syms:
         assert hasattr(L, s), s
     assert set(syms) == set(native.SIGNATURES), "ctypes signatures and header disagree"
+
+
+# C struct in the header -> its ctypes mirror in heu_amd/_native.py
+_STRUCTS = {"heu_task": "HeuTask", "heu_server_config": "HeuServerConfig", "heu_server_counters": "HeuServerStats"}
+_C_SIZES = {"size_t": 8, "uint64_t": 8, "int64_t": 8, "uint32_t": 4, "int32_t": 4}
+
+
+def header_struct_fields(name):
+    """(field, size, align) in declaration order for `typedef struct name {...}`."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct " + name + r" \{(.*?)\} " + name + ";", text, re.S).group(1)
+    out = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        m = re.match(r"(?:const\s+)?(\w+)\s*(\*?)\s*(.*)", decl)
+        ctype, star, names = m.groups()
+        for nm in names.split(","):
+            nm = nm.strip()
+            ptr = bool(star) or nm.startswith("*")
+            size = 8 if ptr else _C_SIZES[ctype]
+            out.append((nm.lstrip("*"), size, size))
+    return out
+
+
+def test_ctypes_structs_match_header(native):
+    import ctypes
+
+    for cname, pyname in _STRUCTS.items():
+        fields = header_struct_fields(cname)
+        cls = getattr(native, pyname)
+        assert [f[0] for f in cls._fields_] == [f[0] for f in fields], cname
+        off = 0
+        for (fname, size, align), (pname, ptype) in zip(fields, cls._fields_):
+            off = (off + align - 1) // align * align
+            assert getattr(cls, pname).offset == off and ctypes.sizeof(ptype) == size, (cname, fname)
+            off += size
+        size = (off + 7) // 8 * 8
+        assert ctypes.sizeof(cls) == size, cname
 
 
 def test_seeded_keygen_matches_oracle(native, oracle_mod):
