@@ -1093,9 +1093,10 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(Mod
 // / 2^60 and d, e likewise after adding the multiple k m of m that makes them
 // divisible.  Limbs stay redundant, in [-2, 2^60 + 1] with the top one whole,
 // so they are the exact integers and no carry crosses more than one lane per
-// round; one lane resolves them at the end.  After the published bound of
-// (49 N + 57) / 17 divsteps (N = bits of m) g = 0 and f = +-gcd(x, m); the
-// kernel checks that and runs further batches if it ever were not so.
+// round; one lane resolves them to check for the end.  After the published
+// bound of (49 N + 57) / 17 divsteps (N = bits of m) g = 0 and f = +-gcd(x, m),
+// and typical inputs get there after about 72% of it; the kernel checks from
+// 70% on and runs further batches past the bound if it ever had to.
 // ---------------------------------------------------------------------------
 constexpr int kSgBits = 60;
 constexpr u64 kSgMask = (1ull << kSgBits) - 1ull;
@@ -1254,9 +1255,13 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
   for (int i = 0; i < 5; ++i) minv *= 2u - m0 * minv;
   const int nbits = 32 * W;
   const int batches = ((49 * nbits + 57) / 17 + kSgBits) / kSgBits;
+  // Random inputs reach g = 0 after 71-73% of the bound (4096 and 6144 bits),
+  // and once g = 0 further batches leave f and d unchanged: check from 70% of
+  // the bound, every 4 batches up to it, every 8 after it.
+  const int first = batches * 7 / 10;
   int delta = 1, done = 0;
   for (;;) {
-    const int todo = done == 0 ? batches : 8;
+    const int todo = done == 0 ? first : done < batches ? 4 : 8;
     for (int bi = 0; bi < todo; ++bi) {
       i64 u, v, q, r;
       delta = sg_divsteps(delta, sg_low64(f), sg_low64(gg), u, v, q, r);
@@ -1275,7 +1280,7 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
       e = ne;
     }
     done += todo;
-    // resolve f, g, d in lane 0
+    // resolve g (then f and d) in lane 0
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       res[0][2 * lane + j] = f.l[j];
@@ -1285,13 +1290,17 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
     __syncthreads();
     if (lane == 0u) {
       i64 *rf = res[0], *rg = res[1], *rd = res[2];
-      const i64 cf = sg_resolve(rf), cg = sg_resolve(rg);
-      bool g_zero = cg == 0 && rg[0] == 0, f_one = cf == 0 && rf[0] == 1;
-      bool f_mone = cf == -1 && rf[0] == (i64)kSgMask;
-      for (int i = 1; i < kSgLimbs; ++i) {
-        g_zero &= rg[i] == 0;
-        f_one &= rf[i] == 0;
-        f_mone &= rf[i] == (i64)kSgMask;
+      bool g_zero = sg_resolve(rg) == 0;
+      for (int i = 0; i < kSgLimbs && g_zero; ++i) g_zero = rg[i] == 0;
+      bool f_one = false, f_mone = false;
+      if (g_zero) {
+        const i64 cf = sg_resolve(rf);
+        f_one = cf == 0 && rf[0] == 1;
+        f_mone = cf == -1 && rf[0] == (i64)kSgMask;
+        for (int i = 1; i < kSgLimbs; ++i) {
+          f_one &= rf[i] == 0;
+          f_mone &= rf[i] == (i64)kSgMask;
+        }
       }
       if (g_zero && (f_one || f_mone)) {
         // |d| as W + 1 words; the sign of d f into the state
