@@ -443,12 +443,11 @@ int plan(Builder& b, heu_ctx* ctx, const std::vector<heu_task>& tasks, uint64_t*
 
 namespace heu_engine {
 
-long worklist_round_units(heu_ctx* ctx, const WorklistShape* shape, int key_bits, bool dec) {
+long worklist_round_units(heu_ctx* ctx, const WorklistShape* shape) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
     cus = 1;
-  const uint32_t mult = ctx->grid_mult[key_bits == 3072][dec ? 1 : 0];
-  return (long)resident_per_cu(ctx, shape) * cus * (long)std::max<uint32_t>(mult, 1);
+  return (long)resident_per_cu(ctx, shape) * cus;
 }
 
 int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, PlanOut* out) {

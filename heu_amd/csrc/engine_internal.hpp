@@ -101,8 +101,10 @@ void ctx_release(heu_ctx* ctx);
 // Checks the tasks and plans them (caller holds ctx->mu).
 int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, PlanOut* out);
 // Work-list units of `shape` that run at once on the whole device: resident
-// workgroups per CU x CUs x the context's grid multiplier (caller holds ctx->mu).
-long worklist_round_units(heu_ctx* ctx, const heu_hip::WorklistShape* shape, int key_bits, bool dec);
+// workgroups per CU x CUs.  A grid multiplier above 1 launches more workgroups
+// than that, but they queue; it does not change what runs at once (caller
+// holds ctx->mu).
+long worklist_round_units(heu_ctx* ctx, const heu_hip::WorklistShape* shape);
 
 }  // namespace heu_engine
 

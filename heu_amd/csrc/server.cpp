@@ -88,7 +88,7 @@ void describe(heu_ctx* ctx, Request& r) {
   const heu_task& t = r.task;
   const bool dec = t.op == HEU_OP_DECRYPT;
   r.shape = dec ? &t.key->dshape().wl : &t.key->ops->wl_n;
-  r.round = std::max<long>(1, worklist_round_units(ctx, r.shape, t.key->key_bits, dec));
+  r.round = std::max<long>(1, worklist_round_units(ctx, r.shape));
   r.gpu = r.shape->groups_per_unit;
   r.factor = dec ? 2 : 1;
   const bool seg = t.op == HEU_OP_SUM || t.op == HEU_OP_DOT_I64;

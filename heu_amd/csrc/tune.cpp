@@ -9,8 +9,9 @@
 //     times the resident workgroups per CU a launch starts (occupancy), for the
 //     key's n^2 shape (ct x scalar) and decryption shape.
 // Every candidate runs once to warm up and then twice; the best time counts.
-// A decryption shape with more lanes replaces the default one only when it is
-// at least 2% faster.
+// A decryption shape with more lanes replaces the default one, and a grid
+// multiplier above 1 replaces 1, only when it is at least 2% faster (the
+// candidates are usually within timing noise of each other).
 // The parameters never change results, only speed.
 #include <hip/hip_runtime.h>
 
@@ -174,7 +175,8 @@ extern "C" int heu_autotune(heu_key* key, uint32_t what, char* report, size_t re
         heu_batch_destroy(b);
         if (rc) return rc;
         js += std::string(mult == 1 ? "" : ", ") + "\"" + std::to_string(mult) + "\": " + fmt("%.3f", tm);
-        if (tm < best_t) {
+        // a wider grid only queues more workgroups: keep it only for a 2% gain
+        if (tm < best_t * (mult == 1 ? 1.0 : 0.98)) {
           best_t = tm;
           best_m = mult;
         }

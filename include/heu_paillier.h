@@ -103,7 +103,8 @@ int heu_key_get_param(const heu_key *key, int param, uint32_t *value);
  * key (synthetic inputs) and keeps the fastest -- the table window (encryption
  * throughput), the decryption lanes per bignum (decryption throughput) and
  * the persistent work-list grid multiplier (occupancy) of the key's shapes,
- * which is set on the key's context.  `what` is a mask of HEU_TUNE_*; report
+ * which is set on the key's context.  A second decryption shape or a
+ * multiplier above 1 is kept only when it is at least 2% faster.  `what` is a mask of HEU_TUNE_*; report
  * (may be NULL) receives a JSON object with every timing and the choices. */
 enum { HEU_TUNE_TABLE = 1, HEU_TUNE_DEC_LANES = 2, HEU_TUNE_GRID = 4, HEU_TUNE_ALL = 7 };
 int heu_autotune(heu_key *key, uint32_t what, char *report, size_t report_len);
@@ -228,8 +229,9 @@ int heu_batch_destroy(heu_batch *batch);
 int heu_batch_stats(const heu_batch *batch, uint64_t *launches, uint64_t *units, size_t *device_bytes);
 
 /* Persistent launch width: workgroups = multiplier x resident workgroups per
- * CU x CUs (multiplier 0 = one workgroup per unit, i.e. not persistent).
- * Applies to batches planned afterwards; the autotuner sets it per shape. */
+ * CU x CUs (at most one per work unit), multiplier in [1, 64], default 1.
+ * Applies to batches planned afterwards; the autotuner sets it per shape.  The
+ * request server budgets by what runs at once (multiplier 1) either way. */
 int heu_ctx_set_grid_multiplier(heu_ctx *ctx, uint32_t key_bits, int decrypt_shape, uint32_t multiplier);
 
 /* ---- request server (continuous batching) ------------------------------ */
