@@ -174,9 +174,11 @@ void plan_segmented(Builder& b, heu_key* key, const heu_task& t, WorkList& wl) {
   const bool dot = t.op == HEU_OP_DOT_I64;
   const size_t segs = t.segments;
   std::vector<long> start(segs), len(segs);
+  long total = 0;
   for (size_t s = 0; s < segs; ++s) {
     start[s] = t.offsets[s];
     len[s] = t.offsets[s + 1] - t.offsets[s];
+    total += len[s];
   }
   struct Level {
     std::vector<Chunk> ch;
@@ -185,7 +187,7 @@ void plan_segmented(Builder& b, heu_key* key, const heu_task& t, WorkList& wl) {
     uint32_t *pos = nullptr, *neg = nullptr;
   };
   std::vector<Level> lv(1);
-  lv[0].ch = make_chunks(start, len, dot ? kReduceChunkDot0 : kReduceChunk0, &lv[0].first, &lv[0].nch);
+  lv[0].ch = make_chunks(start, len, reduce_chunk0(key->ctx, o, dot, total), &lv[0].first, &lv[0].nch);
   for (;;) {
     const Level& prev = lv.back();
     bool done = true;

@@ -80,6 +80,15 @@ std::vector<long> inv_levels(long count) {
   return cnt;
 }
 
+long reduce_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, bool dot, long total) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
+    cus = 1;
+  const long target = 2L * cus * (heu_hip::kBlock / o->n_L);  // groups that fill the chip twice
+  const long cap = dot ? kReduceChunkDot0 : kReduceChunk0;
+  return std::max(1L, std::min(cap, (total + target - 1) / target));
+}
+
 std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, const std::vector<long>& seg_len, long chunk,
                                std::vector<long>* seg_first_chunk, std::vector<long>* seg_nchunks) {
   std::vector<heu_hip::Chunk> ch;
@@ -386,17 +395,9 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
     len[s] = offsets[s + 1] - offsets[s];
     if (len[s] < 0) return fail(HEU_EINVAL, "offsets must be non-decreasing");
   }
-  // Level 0's chunk: the full chunk when the terms fill the chip about twice
-  // over, shorter for fewer terms, so a short vector's level 0 is a short chain
-  // of products rather than a long chain on a few groups.
   long total = 0;
   for (long l : len) total += l;
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
-    cus = 1;
-  const long target = 2L * cus * (heu_hip::kBlock / o->n_L);
-  const long cap = dot ? kReduceChunkDot0 : kReduceChunk0;
-  const long chunk0 = std::max(1L, std::min(cap, (total + target - 1) / target)), chunkU = kReduceChunkUpper;
+  const long chunk0 = reduce_chunk0(ctx, o, dot, total), chunkU = kReduceChunkUpper;
   hipStream_t st = ctx->stream;
   // Plan all levels first (host), then carve scratch once.
   struct Level {

@@ -64,12 +64,17 @@ inline int inv_chunk(size_t level) { return level == 0 ? kInvChunk : kInvChunkUp
 std::vector<long> inv_levels(long count);
 
 // Chunking of segments for a reduction level.  Level 0 reads the inputs in
-// chunks of kReduceChunk0 (kReduceChunkDot0 for dot terms, each a power), which
-// fills the chip for large inputs.  Upper levels see few partials, so a level
-// takes as long as its chunk's sequential products while most of the chip
+// chunks of up to kReduceChunk0 (kReduceChunkDot0 for dot terms, each a power),
+// which fills the chip for large inputs.  Upper levels see few partials, so a
+// level takes as long as its chunk's sequential products while most of the chip
 // idles: chunks of kReduceChunkUpper keep that chain short (128 partials per
 // segment: 4 + 4 + 4 + 2 products instead of 32 + 4).
 constexpr long kReduceChunk0 = 32, kReduceChunkDot0 = 8, kReduceChunkUpper = 4;
+// Level 0's chunk for a reduction of `total` terms: the full chunk when the
+// terms fill the chip about twice over, shorter for fewer, so a short vector's
+// level 0 is a short chain of products rather than a long chain on a few groups.
+// The one-op path, the batch planner and the server's work estimate share it.
+long reduce_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, bool dot, long total);
 std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, const std::vector<long>& seg_len, long chunk,
                                         std::vector<long>* seg_first_chunk, std::vector<long>* seg_nchunks);
 

@@ -95,7 +95,7 @@ void describe(heu_ctx* ctx, Request& r) {
   r.whole = seg || t.count == 0;
   if (r.whole) {
     const long n = seg && t.segments ? (long)(t.offsets[t.segments] - t.offsets[0]) : 0;
-    const long chunk = t.op == HEU_OP_DOT_I64 ? 8 : 32;  // level-0 chunk of the reduction
+    const long chunk = reduce_chunk0(ctx, t.key->ops, t.op == HEU_OP_DOT_I64, n);  // level 0 of the reduction
     r.whole_units = (n / chunk + (long)t.segments + r.gpu - 1) / r.gpu;
     r.n = 1;
     return;
