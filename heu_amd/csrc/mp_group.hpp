@@ -15,12 +15,11 @@
 // iteration, and lives in one lane for at most K iterations: it enters as the
 // lane's top slot holding < 2^29, and leaves as slot 0, whose bits above 29 go
 // to slot 1 of the same lane and whose low 29 bits go to the lane below.  So
-// for K <= 31 no carry step is needed at all.  K = 36 multiplies take a
-// one-step carry over all slots every 31 iterations; K = 36 squarings (whose
-// bound is 27 iterations, below) carry one slot per iteration instead, the one
-// in the middle of the lane, which hands its bits above 32 to the slot above it
-// (one multiply-add by 2^3) and so cuts every slot's life into runs of
-// <= K/2 + 1 iterations between carries.  With R = 2^(29*NL) >= 4m, inputs below 2m give
+// for K <= 31 no carry step is needed at all.  K = 36 products (squarings'
+// bound is 27 iterations, below) carry one slot per iteration, the one in the
+// middle of the lane, which hands its bits above 32 to the slot above it (one
+// multiply-add by 2^3) and so cuts every slot's life into runs of <= K/2 + 1
+// iterations between carries.  With R = 2^(29*NL) >= 4m, inputs below 2m give
 // outputs below 2m, so chained multiplications never need the conditional
 // subtraction; only results leaving the engine are canonicalised.
 //
@@ -212,18 +211,6 @@ __device__ __forceinline__ void get_lds(u32 (&x)[K], const u32* __restrict__ a_l
   for (int t = 0; t < K; ++t) x[t] = a_lds[lid * K + t];
 }
 
-// One carry step over all accumulators: acc[j] = lo(acc[j]) + hi(acc[j-1]).
-// Logical limb j of this lane lives in physical register (j + SH) % K.
-template <int L, int K, int SH>
-__device__ __forceinline__ void norm_step(u64 (&acc)[K], u32 lid) {
-  // top-down, so acc[j-1] is still the old value when acc[j] takes its carry
-  u64 below = grp_shr1_64<L>(acc[(K - 1 + SH) % K] >> kLimbBits, lid);
-#pragma unroll
-  for (int j = K - 1; j >= 1; --j)
-    acc[(j + SH) % K] = (acc[(j + SH) % K] & kLimbMask) + (acc[(j - 1 + SH) % K] >> kLimbBits);
-  acc[SH % K] = (acc[SH % K] & kLimbMask) + below;
-}
-
 // acc = a * b * R^-1 (mod m), lazy: result < 2m when a, b < 2m.
 // a: the group's LDS operand (NL limbs, each < 2^29 + 65);
 // b: this lane's K limbs (each < 2^29 + 65).
@@ -264,22 +251,21 @@ constexpr bool sq_cross_here(int x, int y) {
   return (d >= 1 && d < K / 2) || (d == K / 2 && x < K / 2);
 }
 
-template <int L, int K, bool kSqr, bool kSlotMul = false>
+template <int L, int K, bool kSqr>
 __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
                                           const GMod<L, K>& md) {
   constexpr int kNorm = kSqr ? kNormEverySq : kNormEvery;
-  // Carries when K > kNorm.  Squaring: the middle slot each iteration (runs of
-  // K-1-kMid and kMid+1 iterations).  Multiplication: a carry step over all
-  // slots every kNorm iterations and at the end of each block, or (kSlotMul)
-  // the squaring's per-iteration middle-slot carry.  The per-iteration form
-  // issues fewer instructions, but lets the compiler interleave iterations
-  // further, which spills registers in kernels that also square (K = 36).
+  // Carries when K > kNorm: the slot in the middle of the lane, once per
+  // iteration (runs of K-1-kMid and kMid+1 iterations).  In a multiplication a
+  // scheduling barrier ends each iteration: otherwise the compiler interleaves
+  // iterations until the K = 36 kernels spill (k_dec_half: 724 B of scratch).
+  // With it they spill less than with a full carry step every kNorm iterations.
   constexpr int kMid = K / 2;
-  static_assert(K <= kNorm || (K - 1 - kMid <= kNorm && kMid + 1 <= kNorm), "carry runs too long");
-  // 2^(32-29) for the squaring's carry, hidden from the optimizer, which would
-  // otherwise turn the one multiply-add into a 64-bit shift, two masks and an add.
+  constexpr bool kSlot = K > kNorm;
+  static_assert(!kSlot || (K - 1 - kMid <= kNorm && kMid + 1 <= kNorm), "carry runs too long");
+  // 2^(32-29) for the carry, hidden from the optimizer, which would otherwise
+  // turn the one multiply-add into a 64-bit shift, two masks and an add.
   u32 hi_weight = 1u << (32 - kLimbBits);
-  constexpr bool kSlot = K > kNorm && (kSqr || kSlotMul);
   if constexpr (kSlot) asm volatile("" : "+s"(hi_weight));
   // The limb mask in a register: a DPP operand's partner cannot be a literal.
   u32 mask_v = kLimbMask;
@@ -330,17 +316,16 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
         u64& above = acc[(kMid + 1 + s + 1) % K];
         above = mad64((u32)(x >> 32), hi_weight, above);
         x = (u32)x;
-      } else if constexpr (K > kNorm && ((s + 1) % kNorm == 0 || s == K - 1)) {
-        norm_step<L, K, (s + 1) % K>(acc, lid);
       }
+      if constexpr (kSlot && !kSqr) __builtin_amdgcn_sched_barrier(0);
     });
   }
 }
 
-template <int L, int K, bool kSlotMul = false>
+template <int L, int K>
 __device__ __forceinline__ void mont_mul(u64 (&acc)[K], const u32* __restrict__ a_lds, const u32 (&b)[K],
                                          const GMod<L, K>& md) {
-  mont_core<L, K, false, kSlotMul>(acc, a_lds, b, md);
+  mont_core<L, K, false>(acc, a_lds, b, md);
 }
 
 // acc = a * a * R^-1 (mod m), lazy; b holds this lane's limbs of a_lds.

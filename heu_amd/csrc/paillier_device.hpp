@@ -48,11 +48,10 @@ struct Grp {
   GMod<L, K> md;
   u32 lid;
 
-  // A <- A * b * R^-1 (kSlotMul: the carry form for multiply-only kernels)
-  template <bool kSlotMul = false>
+  // A <- A * b * R^-1
   __device__ __forceinline__ void mul(const u32 (&b)[K]) {
     u64 acc[K];
-    mont_mul<L, K, kSlotMul>(acc, A, b, md);
+    mont_mul<L, K>(acc, A, b, md);
     u32 x[K];
     seminorm<L, K>(acc, x);
     put_lds<L, K>(A, x, lid);
@@ -63,11 +62,10 @@ struct Grp {
     mont_mul<L, K>(acc, slot, b, md);
     seminorm<L, K>(acc, out);
   }
-  template <bool kSlotMul = false>
   __device__ __forceinline__ void mul_global(const u32* __restrict__ src) {
     u32 b[K];
     load_r29<L, K>(b, src, lid);
-    mul<kSlotMul>(b);
+    mul(b);
   }
   __device__ __forceinline__ void square() {
     u32 b[K];
@@ -215,24 +213,21 @@ __device__ __forceinline__ bool r_fits(const u32* __restrict__ rp, int r_words, 
   return ok;
 }
 
-// A <- h_s^r * R mod n^2 from the table (encryptor.cc:26-31 GetRn).  kSlotMul:
-// the per-iteration carry in the table products, for the one-op encryption
-// kernels; the work-list kernel, which also holds every other op, spills with
-// it and keeps the block carry.
-template <bool kSlotMul, int L, int K, int W>
+// A <- h_s^r * R mod n^2 from the table (encryptor.cc:26-31 GetRn).
+template <int L, int K, int W>
 __device__ __forceinline__ void fixed_base_pow(Grp<L, K, W>& g, const PubArgs& P, const u32* __restrict__ rp,
                                                int r_words) {
   constexpr int NL = L * K;
   const long per = 1l << P.unit;
   g.set_global(g.A, P.table + (long)r_digit(rp, r_words, 0, P.unit) * NL);
   for (int w = 1; w < P.windows; ++w)
-    g.template mul_global<kSlotMul>(P.table + (w * per + r_digit(rp, r_words, w * P.unit, P.unit)) * NL);
+    g.mul_global(P.table + (w * per + r_digit(rp, r_words, w * P.unit, P.unit)) * NL);
 }
 
 // ---------------------------------------------------------------------------
 // Encryption (encryptor.cc:35-49): c = (1 + n*m) * h_s^r mod n^2, Z-space out.
 // ---------------------------------------------------------------------------
-template <bool kSlotMul = false, int L, int K, int W>
+template <int L, int K, int W>
 __device__ __forceinline__ void enc_i64_body(Grp<L, K, W>& g, const PubArgs& P, const i64* __restrict__ m,
                                              const u32* __restrict__ r, int r_words, long e, u32* __restrict__ out,
                                              u32* err) {
@@ -241,7 +236,7 @@ __device__ __forceinline__ void enc_i64_body(Grp<L, K, W>& g, const PubArgs& P, 
     if (g.lid == 0) atomicOr(err, kErrRange);
     return;
   }
-  fixed_base_pow<kSlotMul>(g, P, rp, r_words);
+  fixed_base_pow(g, P, rp, r_words);
   u32 gm[K];
   gm_from_i64<L, K>(gm, m[e], P.n29, g.md, g.lid);
   g.mul(gm);  // (h_s^r R) * gm * R^-1 = h_s^r * gm
@@ -256,7 +251,7 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_i64(
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, P.n2, e);
-  if (e < count) enc_i64_body<true>(g, P, m, r, r_words, e, out, err);
+  if (e < count) enc_i64_body(g, P, m, r, r_words, e, out, err);
 }
 
 // Encryption of plaintexts given as two's complement over mw words (|m| <= n/2
@@ -309,14 +304,14 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_encrypt_big(
   }
   norm_full<L, K>(v, x);  // gm = 1 + n m mod n^2 (canonical; m != 0 when negative)
   g.set(g.B, x);
-  fixed_base_pow<true>(g, P, rp, r_words);
+  fixed_base_pow(g, P, rp, r_words);
   g.mul_B();
   g.store_canonical(out + e * W, W);
 }
 
 // Randomize (evaluator.cc:25-28) and EncryptZero (encryptor.cc:33):
 // out = c * h_s^r mod n^2 (c = 1 when ct is null).
-template <bool kSlotMul = false, int L, int K, int W>
+template <int L, int K, int W>
 __device__ __forceinline__ void randomize_body(Grp<L, K, W>& g, const PubArgs& P, const u32* __restrict__ ct,
                                                const u32* __restrict__ r, int r_words, long e, u32* __restrict__ out,
                                                u32* err) {
@@ -329,7 +324,7 @@ __device__ __forceinline__ void randomize_body(Grp<L, K, W>& g, const PubArgs& P
   if (ct) g.load_words(c, ct + e * W, W);
   else one_limbs<L, K>(c, g.lid);
   g.set(g.B, c);
-  fixed_base_pow<kSlotMul>(g, P, rp, r_words);
+  fixed_base_pow(g, P, rp, r_words);
   g.mul_B();
   g.store_canonical(out + e * W, W);
 }
@@ -342,7 +337,7 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_randomize(Pu
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, P.n2, e);
-  if (e < count) randomize_body<true>(g, P, ct, r, r_words, e, out, err);
+  if (e < count) randomize_body(g, P, ct, r, r_words, e, out, err);
 }
 
 // ---------------------------------------------------------------------------
