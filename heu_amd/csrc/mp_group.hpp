@@ -259,7 +259,8 @@ __device__ __forceinline__ void mont_core(u64 (&acc)[K], const u32* __restrict__
   // iteration (runs of K-1-kMid and kMid+1 iterations).  In a multiplication a
   // scheduling barrier ends each iteration: otherwise the compiler interleaves
   // iterations until the K = 36 kernels spill (k_dec_half: 724 B of scratch).
-  // With it they spill less than with a full carry step every kNorm iterations.
+  // With it they spill less than a full carry step over all slots every kNorm
+  // iterations made them spill, and issue fewer instructions.
   constexpr int kMid = K / 2;
   constexpr bool kSlot = K > kNorm;
   static_assert(!kSlot || (K - 1 - kMid <= kNorm && kMid + 1 <= kNorm), "carry runs too long");
