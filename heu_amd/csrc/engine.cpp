@@ -913,21 +913,22 @@ int heu_mul(heu_key* key, const uint32_t* ct, const uint32_t* k_mag, const uint8
 
 int heu_sum(heu_key* key, const uint32_t* ct, const int64_t* offsets, size_t segments, uint32_t* out) {
   KEY_PROLOGUE(key);
-  if (!offsets || !out || (!ct && segments && offsets[segments] > offsets[0])) return fail(HEU_EINVAL, "null argument");
+  if (!offsets || (segments && !out) || (!ct && segments && offsets[segments] > offsets[0])) return fail(HEU_EINVAL, "null argument");
   return segmented(key, ct, nullptr, offsets, segments, out);
 }
 
 int heu_dot_i64(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t* offsets, size_t segments,
                 uint32_t* out) {
   KEY_PROLOGUE(key);
-  if (!offsets || !out || !w) return fail(HEU_EINVAL, "null argument");
+  if (!offsets || (segments && !out) || ((!ct || !w) && segments && offsets[segments] > offsets[0]))
+    return fail(HEU_EINVAL, "null argument");
   return segmented(key, ct, w, offsets, segments, out);
 }
 
 int heu_sum_gather(heu_key* key, const uint32_t* ct, const int64_t* idx, const int64_t* offsets, size_t segments,
                    uint32_t* out) {
   KEY_PROLOGUE(key);
-  if (!offsets || !out || !idx || (!ct && segments && offsets[segments] > offsets[0]))
+  if (!offsets || (segments && !out) || ((!ct || !idx) && segments && offsets[segments] > offsets[0]))
     return fail(HEU_EINVAL, "null argument");
   return segmented(key, ct, nullptr, offsets, segments, out, idx);
 }
@@ -935,7 +936,7 @@ int heu_sum_gather(heu_key* key, const uint32_t* ct, const int64_t* idx, const i
 int heu_dot_gather(heu_key* key, const uint32_t* ct, const int64_t* idx, const int64_t* w, const int64_t* offsets,
                    size_t segments, uint32_t* out) {
   KEY_PROLOGUE(key);
-  if (!offsets || !out || !idx || (segments && offsets[segments] > offsets[0] && (!ct || !w)))
+  if (!offsets || (segments && !out) || (segments && offsets[segments] > offsets[0] && (!ct || !w || !idx)))
     return fail(HEU_EINVAL, "null argument");
   return segmented(key, ct, w, offsets, segments, out, idx);
 }
@@ -943,7 +944,9 @@ int heu_dot_gather(heu_key* key, const uint32_t* ct, const int64_t* idx, const i
 int heu_prefix_mul(heu_key* key, const uint32_t* ct, const int64_t* offsets, size_t segments, uint32_t* out) {
   KEY_PROLOGUE(key);
   if (segments == 0) return HEU_OK;
-  if (!ct || !offsets || !out) return fail(HEU_EINVAL, "null argument");
+  if (!offsets) return fail(HEU_EINVAL, "null argument");
+  if (offsets[segments] == offsets[0]) return HEU_OK;  // no terms: nothing to write
+  if (!ct || !out) return fail(HEU_EINVAL, "null argument");
   std::vector<Chunk> segs(segments);
   for (size_t s = 0; s < segments; ++s) {
     if (offsets[s + 1] < offsets[s]) return fail(HEU_EINVAL, "offsets must be non-decreasing");

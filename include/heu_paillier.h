@@ -19,6 +19,9 @@
  *   - every entry point returns a status; heu_last_error() gives the message.
  *
  * Signed plaintexts cross as two's complement over a stated number of limbs.
+ * Empty work is a successful no-op: count = 0 (or no segments, or segments
+ * with no terms) reads and writes nothing, and pointers to empty arrays may
+ * be NULL.
  */
 #ifndef HEU_PAILLIER_H_
 #define HEU_PAILLIER_H_

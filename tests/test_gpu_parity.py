@@ -290,6 +290,44 @@ def test_config1_workload(eng, oracle_mod, ctx):
     gk.close()
 
 
+def test_empty_inputs(eng, ctx, k2048):
+    # empty spans (an empty HeMatrix, a worker chunk with no elements) are no-ops
+    # that return empty results; a sum or dot over an empty segment is the
+    # identity, Enc-free 1 (numpy/evaluator.cc:355-376 on zero terms)
+    kd, gk, ok = k2048
+    W = gk.ct_words
+    m = np.zeros(0, np.int64)
+    r = eng.seeded_r(5, 5, 0, 256)
+    print("[k2048] empty one-op calls", flush=True)
+    ct = gk.encrypt_i64(m, r)
+    outs = [ct, gk.decrypt(ct, out_words=2), gk.add(ct, ct), gk.mul_i64(ct, m), gk.negate(ct), gk.add_i64(ct, m),
+            gk.randomize(ct, r)]
+    gk.ctx.sync()
+    assert all(o.shape[0] == 0 for o in outs)
+    assert gk.sum(ct, np.zeros(1, np.int64)).shape[0] == 0  # no segments
+    one = gk.sum(ct, np.zeros(2, np.int64))  # one empty segment
+    dot = gk.dot_i64(ct, m, np.zeros(2, np.int64))
+    gk.ctx.sync()
+    assert eng.words_to_int(host(one)[0]) == 1 and eng.words_to_int(host(dot)[0]) == 1
+    print("[k2048] batch and server with empty tasks", flush=True)
+    full_m, full_r = inputs(eng, 5)
+    b = eng.Batch(ctx, graph=False)
+    e_out = b.encrypt_i64(gk, m, r)
+    f_out = b.encrypt_i64(gk, full_m, full_r)
+    b.plan()
+    b.run()
+    gk.ctx.sync()
+    assert e_out.shape[0] == 0 and np.array_equal(host(f_out), ok.encrypt_i64(full_m, full_r))
+    b.close()
+    srv = eng.Server(ctx, workspace_bytes=1 << 30, max_inflight=1)
+    t_e, o_e = srv.encrypt_i64(gk, m, r)
+    t_f, o_f = srv.encrypt_i64(gk, full_m, full_r)
+    srv.wait(t_e, 60000.0)
+    srv.wait(t_f, 60000.0)
+    assert o_e.shape[0] == 0 and np.array_equal(host(o_f), ok.encrypt_i64(full_m, full_r))
+    srv.close()
+
+
 def test_chain_full_size(eng, ctx, k2048):
     # the headline workload at its full size (1,048,576 elements, one op per
     # launch): Dec((Enc(m) (+) Enc(m')) (x) k) = (m + m') k for every element,
