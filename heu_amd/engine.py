@@ -227,6 +227,27 @@ class Key:
             raise ValueError(f"ciphertexts must have shape (N, {self.ct_words})")
         return t.shape[0]
 
+    @staticmethod
+    def _segments(offsets, positions: int) -> np.ndarray:
+        """Host offsets of a segmented reduction, checked so that every device
+        read stays inside the `positions` elements it addresses."""
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        if offs.ndim != 1 or offs.size < 1:
+            raise ValueError("offsets must be a 1-D array of segments + 1 entries")
+        if offs.size > 1 and (offs[0] < 0 or offs[-1] > positions or bool(np.any(np.diff(offs) < 0))):
+            raise ValueError(f"offsets must be non-decreasing within [0, {positions}]")
+        return offs
+
+    @staticmethod
+    def _check_per_position(t: torch.Tensor, offs: np.ndarray, what: str) -> None:
+        if offs.size > 1 and t.shape[0] < offs[-1]:
+            raise ValueError(f"{what} has {t.shape[0]} entries; the segments address {int(offs[-1])}")
+
+    @staticmethod
+    def _check_indices(idx: torch.Tensor, rows: int) -> None:
+        if idx.numel() and (int(idx.min()) < 0 or int(idx.max()) >= rows):
+            raise ValueError(f"indices must lie in [0, {rows})")
+
     # ------------------------------------------------------------- tuning
     @property
     def table_window(self) -> int:
@@ -348,7 +369,7 @@ class Key:
 
     def sum(self, ct, offsets) -> torch.Tensor:
         ct = self._dev(ct, torch.int32)
-        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        offs = self._segments(offsets, self._rows(ct))
         segs = len(offs) - 1
         out = self._ct_out(segs)
         check(lib().heu_sum(self._h, _ptr(ct), _ptr(offs), segs, _ptr(out)))
@@ -356,7 +377,8 @@ class Key:
 
     def dot_i64(self, ct, w, offsets) -> torch.Tensor:
         ct, w = self._dev(ct, torch.int32), self._dev(w, torch.int64)
-        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        offs = self._segments(offsets, self._rows(ct))
+        self._check_per_position(w, offs, "w")
         segs = len(offs) - 1
         out = self._ct_out(segs)
         check(lib().heu_dot_i64(self._h, _ptr(ct), _ptr(w), _ptr(offs), segs, _ptr(out)))
@@ -365,7 +387,8 @@ class Key:
     def sum_gather(self, ct, idx, offsets) -> torch.Tensor:
         """out[s] = prod ct[idx[j]] over positions j of segment s."""
         ct, idx = self._dev(ct, torch.int32), self._dev(idx, torch.int64)
-        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        offs = self._segments(offsets, idx.shape[0])
+        self._check_indices(idx[: int(offs[-1])], self._rows(ct))
         segs = len(offs) - 1
         out = self._ct_out(segs)
         check(lib().heu_sum_gather(self._h, _ptr(ct), _ptr(idx), _ptr(offs), segs, _ptr(out)))
@@ -374,7 +397,9 @@ class Key:
     def dot_gather(self, ct, idx, w, offsets) -> torch.Tensor:
         """out[s] = prod ct[idx[j]]^w[j] over positions j of segment s."""
         ct, idx, w = self._dev(ct, torch.int32), self._dev(idx, torch.int64), self._dev(w, torch.int64)
-        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        offs = self._segments(offsets, idx.shape[0])
+        self._check_indices(idx[: int(offs[-1])], self._rows(ct))
+        self._check_per_position(w, offs, "w")
         segs = len(offs) - 1
         out = self._ct_out(segs)
         check(lib().heu_dot_gather(self._h, _ptr(ct), _ptr(idx), _ptr(w), _ptr(offs), segs, _ptr(out)))
@@ -383,7 +408,7 @@ class Key:
     def prefix_mul(self, ct, offsets) -> torch.Tensor:
         """Inclusive prefix products within each segment."""
         ct = self._dev(ct, torch.int32)
-        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        offs = self._segments(offsets, self._rows(ct))
         out = self._ct_out(self._rows(ct))
         check(lib().heu_prefix_mul(self._h, _ptr(ct), _ptr(offs), len(offs) - 1, _ptr(out)))
         return out
@@ -453,13 +478,14 @@ class _TaskOps:
 
     def sum(self, key: "Key", ct, offsets) -> torch.Tensor:
         ct = key._dev(ct, torch.int32)
-        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        offs = key._segments(offsets, key._rows(ct))
         segs = len(offs) - 1
         return self._record(key, _native.HEU_OP_SUM, ct, None, 0, key._ct_out(segs), offsets=offs, segments=segs)
 
     def dot_i64(self, key: "Key", ct, w, offsets) -> torch.Tensor:
         ct, w = key._dev(ct, torch.int32), key._dev(w, torch.int64)
-        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        offs = key._segments(offsets, key._rows(ct))
+        key._check_per_position(w, offs, "w")
         segs = len(offs) - 1
         return self._record(key, _native.HEU_OP_DOT_I64, ct, w, 0, key._ct_out(segs), offsets=offs, segments=segs)
 

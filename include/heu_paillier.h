@@ -159,7 +159,11 @@ int heu_dot_i64(heu_key *key, const uint32_t *ct, const int64_t *w, const int64_
  * segment s, j in [offsets[s], offsets[s+1]) (HOST offsets), address
  * ciphertext idx[j] (device int64 indices into ct):
  *   heu_sum_gather: out[s] = prod ct[idx[j]]            (1 for an empty segment)
- *   heu_dot_gather: out[s] = prod ct[idx[j]]^w[j]       (w: device int64, per position) */
+ *   heu_dot_gather: out[s] = prod ct[idx[j]]^w[j]       (w: device int64, per position)
+ * The engine does not know how many ciphertexts ct holds.  So the caller
+ * guarantees that every idx[j] addressed lies within it, as it does for the
+ * offsets of heu_sum / heu_dot_i64 / heu_prefix_mul.  heu_amd/engine.py
+ * checks both on the host before it calls. */
 int heu_sum_gather(heu_key *key, const uint32_t *ct, const int64_t *idx, const int64_t *offsets, size_t segments,
                    uint32_t *out);
 int heu_dot_gather(heu_key *key, const uint32_t *ct, const int64_t *idx, const int64_t *w, const int64_t *offsets,

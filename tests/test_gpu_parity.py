@@ -263,6 +263,20 @@ def test_errors(eng, ctx, k2048):
     with pytest.raises(HeuError):
         ctx.sync()
     ctx.sync()  # flags cleared
+    # segmented reductions: offsets, indices and weights that would send a
+    # device read past its array are refused on the host, before any launch
+    w = np.ones(4, np.int64)
+    bad = [lambda: gk.sum(ct, [0, 5]), lambda: gk.sum(ct, [2, 1]), lambda: gk.prefix_mul(ct, [0, 5]),
+           lambda: gk.dot_i64(ct, w[:3], [0, 4]), lambda: gk.sum_gather(ct, np.array([0, 4]), [0, 2]),
+           lambda: gk.sum_gather(ct, np.array([0, -1]), [0, 2]), lambda: gk.sum_gather(ct, np.array([0]), [0, 2]),
+           lambda: gk.dot_gather(ct, np.array([0, 1]), w[:1], [0, 2])]
+    for i, call in enumerate(bad):
+        with pytest.raises(ValueError):
+            call()
+            print("not refused:", i, flush=True)
+    ok_sum = gk.sum_gather(ct, np.array([3, 0, 3]), [0, 3])
+    ctx.sync()
+    assert np.array_equal(host(ok_sum), ok.dot_i64(np.stack([host(ct)[i] for i in (3, 0, 3)]), None, [0, 3]))
 
 
 def test_config1_workload(eng, oracle_mod, ctx):
