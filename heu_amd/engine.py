@@ -227,6 +227,20 @@ class Key:
             raise ValueError(f"ciphertexts must have shape (N, {self.ct_words})")
         return t.shape[0]
 
+    def _check_rows(self, n: int, cts=(), rows=(), rands=()) -> None:
+        """Every further operand holds exactly n rows, so each device read stays
+        inside its array: ciphertext arrays (n, ct_words), per-element arrays
+        (n,), randomness (n, r_words)."""
+        for t in cts:
+            if self._rows(t) != n:
+                raise ValueError(f"ciphertext operands must have {n} rows, got {t.shape[0]}")
+        for t in rows:
+            if t.dim() != 1 or t.shape[0] != n:
+                raise ValueError(f"per-element operands must have shape ({n},), got {tuple(t.shape)}")
+        for t in rands:
+            if t.dim() != 2 or t.shape[0] != n or t.shape[1] == 0:
+                raise ValueError(f"r must have shape ({n}, r_words), got {tuple(t.shape)}")
+
     @staticmethod
     def _segments(offsets, positions: int) -> np.ndarray:
         """Host offsets of a segmented reduction, checked so that every device
@@ -282,8 +296,7 @@ class Key:
     def encrypt_i64(self, m, r) -> torch.Tensor:
         m = self._dev(m, torch.int64)
         r = self._dev(r, torch.int32)
-        if r.dim() != 2 or r.shape[0] != m.shape[0]:
-            raise ValueError("r must have shape (N, r_words)")
+        self._check_rows(m.shape[0], rows=(m,), rands=(r,))
         out = self._ct_out(m.shape[0])
         check(lib().heu_encrypt_i64(self._h, _ptr(m), _ptr(r), r.shape[1], m.shape[0], _ptr(out)))
         return out
@@ -294,6 +307,7 @@ class Key:
         mh = ints_to_words(values, mw, signed=True)
         md = self._dev(mh, torch.int32)
         r = self._dev(r, torch.int32)
+        self._check_rows(len(values), rands=(r,))
         out = self._ct_out(len(values))
         check(lib().heu_encrypt(self._h, _ptr(md), _ptr(mh), mw, _ptr(r), r.shape[1], len(values), _ptr(out)))
         return out
@@ -307,6 +321,7 @@ class Key:
             return out
         ct = self._dev(ct, torch.int32)
         n = self._rows(ct)
+        self._check_rows(n, rands=(r,))
         out = self._ct_out(n)
         check(lib().heu_randomize(self._h, _ptr(ct), _ptr(r), r.shape[1], n, _ptr(out)))
         return out
@@ -323,6 +338,7 @@ class Key:
     def add(self, a, b) -> torch.Tensor:
         a, b = self._dev(a, torch.int32), self._dev(b, torch.int32)
         n = self._rows(a)
+        self._check_rows(n, cts=(b,))
         out = self._ct_out(n)
         check(lib().heu_add(self._h, _ptr(a), _ptr(b), n, _ptr(out)))
         return out
@@ -330,6 +346,7 @@ class Key:
     def add_i64(self, a, p) -> torch.Tensor:
         a, p = self._dev(a, torch.int32), self._dev(p, torch.int64)
         n = self._rows(a)
+        self._check_rows(n, rows=(p,))
         out = self._ct_out(n)
         check(lib().heu_add_i64(self._h, _ptr(a), _ptr(p), n, _ptr(out)))
         return out
@@ -337,6 +354,7 @@ class Key:
     def sub(self, a, b) -> torch.Tensor:
         a, b = self._dev(a, torch.int32), self._dev(b, torch.int32)
         n = self._rows(a)
+        self._check_rows(n, cts=(b,))
         out = self._ct_out(n)
         check(lib().heu_sub(self._h, _ptr(a), _ptr(b), n, _ptr(out)))
         return out
@@ -351,6 +369,7 @@ class Key:
     def mul_i64(self, a, k) -> torch.Tensor:
         a, k = self._dev(a, torch.int32), self._dev(k, torch.int64)
         n = self._rows(a)
+        self._check_rows(n, rows=(k,))
         out = self._ct_out(n)
         check(lib().heu_mul_i64(self._h, _ptr(a), _ptr(k), n, _ptr(out)))
         return out
@@ -359,6 +378,8 @@ class Key:
         """ct x scalar for Python ints of any size (evaluator.cc:99-123)."""
         a = self._dev(a, torch.int32)
         n = self._rows(a)
+        if len(scalars) != n:
+            raise ValueError(f"need {n} scalars, got {len(scalars)}")
         mags = [abs(int(v)) for v in scalars]
         kw = max(1, max((m.bit_length() + 31) // 32 for m in mags) if mags else 1)
         kmag = self._dev(ints_to_words(mags, kw), torch.int32)
@@ -442,6 +463,7 @@ class _TaskOps:
     def encrypt_i64(self, key: "Key", m, r, out: torch.Tensor | None = None) -> torch.Tensor:
         m, r = key._dev(m, torch.int64), key._dev(r, torch.int32)
         n = m.shape[0]
+        key._check_rows(n, rows=(m,), rands=(r,))
         if out is None:
             out = key._ct_out(n)
         elif not out.is_contiguous() or tuple(out.shape) != (n, key.ct_words) or out.dtype != torch.int32:
@@ -452,21 +474,25 @@ class _TaskOps:
         r = key._dev(r, torch.int32)
         ct = None if ct is None else key._dev(ct, torch.int32)
         n = r.shape[0]
+        key._check_rows(n, cts=() if ct is None else (ct,), rands=(r,))
         return self._record(key, _native.HEU_OP_RANDOMIZE, ct, r, n, key._ct_out(n), aux=r.shape[1])
 
     def add(self, key: "Key", a, b) -> torch.Tensor:
         a, b = key._dev(a, torch.int32), key._dev(b, torch.int32)
         n = key._rows(a)
+        key._check_rows(n, cts=(b,))
         return self._record(key, _native.HEU_OP_ADD, a, b, n, key._ct_out(n))
 
     def add_i64(self, key: "Key", a, p) -> torch.Tensor:
         a, p = key._dev(a, torch.int32), key._dev(p, torch.int64)
         n = key._rows(a)
+        key._check_rows(n, rows=(p,))
         return self._record(key, _native.HEU_OP_ADD_I64, a, p, n, key._ct_out(n))
 
     def mul_i64(self, key: "Key", a, k) -> torch.Tensor:
         a, k = key._dev(a, torch.int32), key._dev(k, torch.int64)
         n = key._rows(a)
+        key._check_rows(n, rows=(k,))
         return self._record(key, _native.HEU_OP_MUL_I64, a, k, n, key._ct_out(n))
 
     def decrypt(self, key: "Key", ct, out_words: int | None = None) -> torch.Tensor:

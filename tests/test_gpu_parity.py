@@ -263,13 +263,17 @@ def test_errors(eng, ctx, k2048):
     with pytest.raises(HeuError):
         ctx.sync()
     ctx.sync()  # flags cleared
-    # segmented reductions: offsets, indices and weights that would send a
-    # device read past its array are refused on the host, before any launch
+    # segmented reductions (offsets, indices, weights) and elementwise operands
+    # that would send a device read past its array are refused on the host,
+    # before any launch
     w = np.ones(4, np.int64)
     bad = [lambda: gk.sum(ct, [0, 5]), lambda: gk.sum(ct, [2, 1]), lambda: gk.prefix_mul(ct, [0, 5]),
            lambda: gk.dot_i64(ct, w[:3], [0, 4]), lambda: gk.sum_gather(ct, np.array([0, 4]), [0, 2]),
            lambda: gk.sum_gather(ct, np.array([0, -1]), [0, 2]), lambda: gk.sum_gather(ct, np.array([0]), [0, 2]),
-           lambda: gk.dot_gather(ct, np.array([0, 1]), w[:1], [0, 2])]
+           lambda: gk.dot_gather(ct, np.array([0, 1]), w[:1], [0, 2]),
+           # elementwise second operands of the wrong length
+           lambda: gk.add(ct, ct[:3]), lambda: gk.sub(ct[:3], ct), lambda: gk.mul_i64(ct, w[:3]),
+           lambda: gk.add_i64(ct, w[:2]), lambda: gk.randomize(ct, r[:3]), lambda: gk.encrypt_i64(m, r[:3])]
     for i, call in enumerate(bad):
         with pytest.raises(ValueError):
             call()
