@@ -494,6 +494,11 @@ void SpanRates(const Keys& k, size_t count) {
   bool same = d.size() == count;
   for (size_t i = 0; same && i < count; ++i) same = d[i] == m[i];
   EXPECT_TRUE(same);
+  // every element of the sums and products as well
+  const std::vector<BigInt> ds = dec.Decrypt(cs(cptrs(s))), dp = dec.Decrypt(cs(cptrs(p)));
+  bool sums = ds.size() == count && dp.size() == count;
+  for (size_t i = 0; sums && i < count; ++i) sums = ds[i] == m[i] + m[i] && dp[i] == m[i] * w[i];
+  EXPECT_TRUE(sums);
   std::printf("  span rates, %zu elements (elements/s, host spans, PCIe included): encrypt %.0f, add %.0f, "
               "mul %.0f, decrypt %.0f\n",
               count, count / te, count / ta, count / tm, count / td);
