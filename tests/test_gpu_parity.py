@@ -311,7 +311,7 @@ def test_config1_workload(eng, oracle_mod, ctx):
 def test_empty_inputs(eng, ctx, k2048):
     # empty spans (an empty HeMatrix, a worker chunk with no elements) are no-ops
     # that return empty results; a sum or dot over an empty segment is the
-    # identity, Enc-free 1 (numpy/evaluator.cc:355-376 on zero terms)
+    # identity 1, which decrypts to 0 (numpy/evaluator.cc:355-376 on zero terms)
     kd, gk, ok = k2048
     W = gk.ct_words
     m = np.zeros(0, np.int64)
