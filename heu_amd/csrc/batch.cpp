@@ -577,11 +577,17 @@ int heu_batch_run(heu_batch* batch) {
     hipGraph_t g = nullptr;
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(batch->cap_stream, &g);
+    if (g && (qe != hipSuccess || ce != hipSuccess || le != hipSuccess)) (void)hipGraphDestroy(g);
     HIP_TRY(qe);
     HIP_TRY(ce);
     HIP_TRY(le);
+    const hipError_t ie = hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      batch->exec = nullptr;
+      HIP_TRY(ie);
+    }
     batch->graph = g;
-    HIP_TRY(hipGraphInstantiate(&batch->exec, batch->graph, nullptr, nullptr, 0));
   }
   HIP_TRY(hipGraphLaunch(batch->exec, ctx->stream));
   return HEU_OK;

@@ -487,13 +487,17 @@ int heu_ctx_create(int device, heu_ctx** out) {
   HIP_TRY(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(HEU_EINVAL, "device %d not present (%d devices)", device, n);
   HIP_TRY(hipSetDevice(device));
-  auto ctx = std::make_unique<heu_ctx>();
+  auto ctx = new heu_ctx();
   ctx->device = device;
-  HIP_TRY(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+  hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
   ctx->stream = ctx->own_stream;
-  HIP_TRY(hipMalloc(&ctx->d_err, sizeof(uint32_t)));
-  HIP_TRY(hipMemset(ctx->d_err, 0, sizeof(uint32_t)));
-  *out = ctx.release();
+  if (e == hipSuccess) e = hipMalloc(&ctx->d_err, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(ctx->d_err, 0, sizeof(uint32_t));
+  if (e != hipSuccess) {
+    ctx_free(ctx);
+    return fail(e == hipErrorOutOfMemory ? HEU_ENOMEM : HEU_EDEVICE, "context set-up: %s", hipGetErrorString(e));
+  }
+  *out = ctx;
   return HEU_OK;
 }
 

@@ -457,6 +457,19 @@ void heu_server::run() {
   }
 }
 
+// Waits for and frees every slot's stream, event and pinned staging.
+static void release_slots(heu_server* srv) {
+  for (Slot& s : srv->slots) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.pinned) (void)hipHostFree(s.pinned);
+    s.stream = nullptr;
+    s.done = nullptr;
+    s.pinned = nullptr;
+  }
+}
+
 extern "C" {
 
 int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** out) {
@@ -477,8 +490,12 @@ int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** o
   srv->arena.init(srv->ws.bytes);
   srv->slots.resize(srv->cfg.max_inflight);
   for (Slot& s : srv->slots) {
-    HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      release_slots(srv.get());
+      return fail(HEU_EDEVICE, "server stream or event: %s", hipGetErrorString(e));
+    }
   }
   ctx_retain(ctx);  // the server holds the context
   heu_server* raw = srv.release();
@@ -557,12 +574,7 @@ int heu_server_destroy(heu_server* srv) {
   srv->cv_work.notify_all();
   if (srv->worker.joinable()) srv->worker.join();
   (void)hipSetDevice(srv->ctx->device);
-  for (Slot& s : srv->slots) {
-    if (s.stream) (void)hipStreamSynchronize(s.stream);
-    if (s.done) (void)hipEventDestroy(s.done);
-    if (s.stream) (void)hipStreamDestroy(s.stream);
-    if (s.pinned) (void)hipHostFree(s.pinned);
-  }
+  release_slots(srv);
   heu_ctx* ctx = srv->ctx;
   delete srv;
   ctx_release(ctx);
