@@ -637,7 +637,8 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
     BN_sub_word(pm1, 1);
     BN_copy(qm1, q);
     BN_sub_word(qm1, 1);
-    auto sp = window_schedule(pm1, heu_hip::kDecWindowBits), sq = window_schedule(qm1, heu_hip::kDecWindowBits);
+    const int dec_w = heu_hip::dec_window_bits(key->key_bits);
+    auto sp = window_schedule(pm1, dec_w), sq = window_schedule(qm1, dec_w);
     key->nsched_p = (int)sp.size();
     key->nsched_q = (int)sq.size();
     if ((rc = upload(key->sched_p, sp, st, tb)) || (rc = upload(key->sched_q, sq, st, tb))) return rc;
@@ -810,7 +811,7 @@ int heu_decrypt(heu_key* key, const uint32_t* ct, size_t count, uint32_t* out, u
   const size_t HWB = (size_t)key->ct_words / 2 * 4;
   Scratch sc(ctx);
   size_t halves = sc.add(2 * (size_t)slice * HWB);
-  size_t tab = sc.add(2 * (size_t)slice * ((size_t)key->nl_p() * 4 << (heu_hip::kDecWindowBits - 1)));
+  size_t tab = sc.add(2 * (size_t)slice * ((size_t)key->nl_p() * 4 << (heu_hip::dec_window_bits(key->key_bits) - 1)));
   int rc = sc.commit();
   if (rc) return rc;
   const DecArgs D = key->dec();

@@ -15,7 +15,9 @@
 namespace heu_hip {
 
 constexpr int kPowWin = kPowWindowBits;
-constexpr int kDecWin = kDecWindowBits;
+// Decryption window for a ciphertext of W words (key bits = 16 W).
+template <int W>
+constexpr int kDecWin = dec_window_bits(16 * W);
 
 template <int L, int K, int W>
 struct Cfg {
@@ -549,7 +551,7 @@ __device__ __forceinline__ void dec_half_body(Grp<L, K, W>& g, const ModArgs& mo
     g.mul_to(x2, g.A, b);  // x^2
     g.set(g.B, x2);
   }
-  for (int i = 1; i < (1 << (kDecWin - 1)); ++i) {
+  for (int i = 1; i < (1 << (kDecWin<W> - 1)); ++i) {
     g.mul_B();
     g.save(T + i * NL, g.A);
   }
@@ -579,7 +581,7 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_dec_half(Dec
   auto g = grp_setup<L, K, W>(lds, mod, e);
   if (e >= count) return;
   dec_half_body(g, mod, h ? D.sched_q : D.sched_p, h ? D.nsched_q : D.nsched_p, ct + e * W,
-                tab + (h * count + e) * (long)(NL << (kDecWin - 1)), halves + (h * count + e) * HW);
+                tab + (h * count + e) * (long)(NL << (kDecWin<W> - 1)), halves + (h * count + e) * HW);
 }
 
 // ---------------------------------------------------------------------------
@@ -1412,7 +1414,7 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_worklist(con
     if (e < it.count) {
       if constexpr (kDec) {
         dec_half_body(g, it.pub.n2, static_cast<const u32*>(it.c), it.aux, static_cast<const u32*>(it.a) + e * W,
-                      slot_tab + slot * (long)(NL << (kDecWin - 1)), static_cast<u32*>(it.out) + e * (W / 2));
+                      slot_tab + slot * (long)(NL << (kDecWin<W> - 1)), static_cast<u32*>(it.out) + e * (W / 2));
       } else {
         u32* T = slot_tab + slot * (long)(NL << kPowWin);
         switch (it.op) {
@@ -1594,7 +1596,7 @@ struct Launch {
     w.launch = worklist<kDec>;
     w.resident = resident<kDec>;
     w.groups_per_unit = kBlock / L;
-    w.slot_table_words = (long)(L * K) << (kDec ? kDecWin - 1 : kPowWin);
+    w.slot_table_words = (long)(L * K) << (kDec ? kDecWin<W> - 1 : kPowWin);
     return w;
   }
 };
