@@ -626,7 +626,3 @@ class Server(_TaskOps):
             self.close()
         except Exception:
             pass
-
-
-def to_host_words(t: torch.Tensor) -> np.ndarray:
-    return t.cpu().numpy().view(np.uint32)
