@@ -1118,44 +1118,49 @@ __device__ __forceinline__ u64 sg_low64(const SgNum& x) {
 
 // 60 divsteps (delta, f, g) -> (1 - delta, g, (g - f) / 2) if delta > 0 and g
 // is odd, else (1 + delta, f, (g + (g mod 2) f) / 2), on the low 64 bits, with
-// the matrix such that f' 2^60 = u f + v g and g' 2^60 = q f + r g.  A run of
-// z even steps is taken at once (z = trailing zeros of g): g >>= z, u, v <<= z.
+// the matrix such that f' 2^60 = u f + v g and g' 2^60 = q f + r g.  Several
+// divsteps are taken per iteration (the variable-time form of the method, as
+// in P. Wuille's safegcd notes for libsecp256k1): a run of z even steps at
+// once (z = trailing zeros of g: g >>= z, u, v <<= z), then, with g odd, the
+// next `limit` steps, which cannot swap while delta stays <= 0, as one
+// elimination g += w f with w = -g / f mod 2^limit (limit <= 6; 1 / f mod 64
+// is f (2 - f^2) for odd f).  A swap step first exchanges the rows as
+// (f, g) <- (g, -f).  The result is the same matrix and delta as one divstep
+// at a time, in about half the iterations.
 __device__ __forceinline__ int sg_divsteps(int delta, u64 f, u64 g, i64& u, i64& v, i64& q, i64& r) {
-  u = 1;
-  v = 0;
-  q = 0;
-  r = 1;
+  u64 uu = 1, vv = 0, qq = 0, rr = 1;
   int i = kSgBits;
-  while (i > 0) {
-    int z = g ? __builtin_ctzll(g) : i;
-    z = z < i ? z : i;
-    g = (u64)((i64)g >> z);
-    u = (i64)((u64)u << z);
-    v = (i64)((u64)v << z);
+  for (;;) {
+    const int z = __builtin_ctzll(g | (~0ull << i));  // at most i
+    g >>= z;
+    uu <<= z;
+    vv <<= z;
     delta += z;
     i -= z;
     if (i == 0) break;
-    if (delta > 0) {  // g odd
-      delta = 1 - delta;
-      const u64 t = f;
+    if (delta > 0) {  // g odd: swap
+      delta = -delta;
+      u64 t = f;
       f = g;
-      g = g - t;
-      const i64 nu = 2 * q, nv = 2 * r;
-      q -= u;
-      r -= v;
-      u = nu;
-      v = nv;
-    } else {
-      delta = 1 + delta;
-      g = g + f;
-      q += u;
-      r += v;
-      u *= 2;
-      v *= 2;
+      g = 0u - t;
+      t = uu;
+      uu = qq;
+      qq = 0u - t;
+      t = vv;
+      vv = rr;
+      rr = 0u - t;
     }
-    g = (u64)((i64)g >> 1);
-    --i;
+    const int limit = 1 - delta < i ? 1 - delta : i;  // >= 1
+    const u32 m = (u32)((~0ull >> (64 - limit)) & 63u);
+    const u32 f32 = (u32)f, w = (f32 * (u32)g * (f32 * f32 - 2u)) & m;
+    g += f * w;
+    qq += uu * w;
+    rr += vv * w;
   }
+  u = (i64)uu;
+  v = (i64)vv;
+  q = (i64)qq;
+  r = (i64)rr;
   return delta;
 }
 
