@@ -123,7 +123,8 @@ void plan_invert(Builder& b, heu_key* key, const uint32_t* in, const uint8_t* se
   const Ops* o = key->ops;
   const ModArgs mod = key->mod_n2();
   const size_t NLB = (size_t)key->nl_n * 4;
-  const std::vector<long> cnt = inv_levels(count);
+  const long chunk0 = inv_chunk0(key->ctx, o, count);
+  const std::vector<long> cnt = inv_levels(count, chunk0);
   const int levels = (int)cnt.size() - 1;
   // selected elements (in place) run from a packed index list
   heu_hip::InvSel s0{selected ? sel : nullptr, nullptr, nullptr};
@@ -145,7 +146,7 @@ void plan_invert(Builder& b, heu_key* key, const uint32_t* in, const uint8_t* se
     const heu_hip::InvSel s = l == 0 ? s0 : none;
     uint32_t *p = pre[l], *t = tot[l + 1];
     const long n = cnt[l];
-    const int chunk = inv_chunk(l);
+    const int chunk = inv_chunk(l, chunk0);
     b.then([=](hipStream_t st) { o->inv_prefix(st, mod, l == 0, src, s, n, chunk, p, t); });
   }
   const uint32_t* n2w = key->n2_words.as<uint32_t>();
@@ -160,7 +161,7 @@ void plan_invert(Builder& b, heu_key* key, const uint32_t* in, const uint8_t* se
     uint32_t* dst = l == 0 ? out : inv[l];
     uint32_t *p = pre[l], *ti = inv[l + 1];
     const long n = cnt[l];
-    const int chunk = inv_chunk(l);
+    const int chunk = inv_chunk(l, chunk0);
     b.then([=](hipStream_t st) { o->inv_down(st, mod, l == 0, src, s, n, chunk, p, ti, dst); });
   }
 }

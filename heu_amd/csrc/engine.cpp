@@ -71,22 +71,32 @@ void ctx_release(heu_ctx* ctx) {
   if (free_now) ctx_free(ctx);
 }
 
-std::vector<long> inv_levels(long count) {
+std::vector<long> inv_levels(long count, long chunk0) {
   std::vector<long> cnt{count};
   while (cnt.size() < 2 || cnt.back() > 1) {
-    const long c = inv_chunk(cnt.size() - 1);
+    const long c = inv_chunk(cnt.size() - 1, chunk0);
     cnt.push_back((cnt.back() + c - 1) / c);
   }
   return cnt;
 }
 
-long reduce_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, bool dot, long total) {
+// Groups of the n^2 shape that fill the chip about twice over.
+static long fill_groups(heu_ctx* ctx, const heu_hip::Ops* o) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
     cus = 1;
-  const long target = 2L * cus * (heu_hip::kBlock / o->n_L);  // groups that fill the chip twice
+  return 2L * cus * (heu_hip::kBlock / o->n_L);
+}
+
+long reduce_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, bool dot, long total) {
+  const long target = fill_groups(ctx, o);
   const long cap = dot ? kReduceChunkDot0 : kReduceChunk0;
   return std::max(1L, std::min(cap, (total + target - 1) / target));
+}
+
+long inv_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, long count) {
+  const long target = fill_groups(ctx, o);
+  return std::max(kInvChunkUpper, std::min(kInvChunk0, (count + target - 1) / target));
 }
 
 std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, const std::vector<long>& seg_len, long chunk,
@@ -317,7 +327,7 @@ int check_key(heu_key* key) {
 // Scratch bytes batch_invert carves for `count` elements.
 size_t inv_scratch_bytes(const heu_key* key, long count) {
   const size_t NLB = (size_t)key->nl_n * 4;
-  auto cnt = inv_levels(count);
+  auto cnt = inv_levels(count, inv_chunk0(key->ctx, key->ops, count));
   size_t b = (((size_t)count * 4 + 255) & ~(size_t)255) + 256;  // index list and its count
   for (size_t l = 0; l < cnt.size(); ++l) {
     size_t part = (cnt[l] * NLB + 255) & ~(size_t)255;
@@ -332,7 +342,8 @@ int batch_invert(heu_key* key, const uint32_t* in, const uint8_t* sel, long coun
   const ModArgs mod = key->mod_n2();
   const size_t NLB = (size_t)key->nl_n * 4;
   if (count <= 0) return HEU_OK;
-  std::vector<long> cnt = inv_levels(count);
+  const long chunk0 = inv_chunk0(ctx, o, count);
+  std::vector<long> cnt = inv_levels(count, chunk0);
   const int levels = (int)cnt.size() - 1;  // cnt[levels] == 1
   Scratch sc(ctx, base);
   // selected elements (always in place here) run from a packed index list
@@ -356,7 +367,7 @@ int batch_invert(heu_key* key, const uint32_t* in, const uint8_t* sel, long coun
   // up: level l reduces cnt[l] values into cnt[l+1] totals
   for (int l = 0; l < levels; ++l) {
     const uint32_t* src = l == 0 ? in : sc.at<uint32_t>(tot[l]);
-    LAUNCH(ctx, "inv_prefix", o->inv_prefix(st, mod, l == 0, src, l == 0 ? s0 : none, cnt[l], inv_chunk(l),
+    LAUNCH(ctx, "inv_prefix", o->inv_prefix(st, mod, l == 0, src, l == 0 ? s0 : none, cnt[l], inv_chunk(l, chunk0),
                                             sc.at<uint32_t>(pre[l]), sc.at<uint32_t>(tot[l + 1])));
   }
   // the single top value
@@ -366,7 +377,7 @@ int batch_invert(heu_key* key, const uint32_t* in, const uint8_t* sel, long coun
   for (int l = levels - 1; l >= 0; --l) {
     const uint32_t* src = l == 0 ? in : sc.at<uint32_t>(tot[l]);
     uint32_t* dst = l == 0 ? out : sc.at<uint32_t>(inv[l]);
-    LAUNCH(ctx, "inv_down", o->inv_down(st, mod, l == 0, src, l == 0 ? s0 : none, cnt[l], inv_chunk(l),
+    LAUNCH(ctx, "inv_down", o->inv_down(st, mod, l == 0, src, l == 0 ? s0 : none, cnt[l], inv_chunk(l, chunk0),
                                         sc.at<uint32_t>(pre[l]), sc.at<uint32_t>(inv[l + 1]), dst));
   }
   HIP_TRY(hipGetLastError());

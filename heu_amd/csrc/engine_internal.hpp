@@ -54,14 +54,18 @@ struct DevBuf {
 };
 
 // Batch inversion (Montgomery's trick): level 0 multiplies the inputs in
-// chunks of kInvChunk (real work, the whole chip); upper levels see few values,
-// where a level's time is its chunk's chain of sequential products, so they use
-// chunks of kInvChunkUpper.
-constexpr int kInvChunk = 32, kInvChunkUpper = 4;
-inline int inv_chunk(size_t level) { return level == 0 ? kInvChunk : kInvChunkUpper; }
+// chunks of up to kInvChunk0, which fills the chip for large batches.  A batch
+// that would not fill the chip about twice over gets shorter level-0 chunks
+// (inv_chunk0, at least kInvChunkUpper), so a short batch is a short chain of
+// products rather than a long chain on a few groups.  Upper levels see few
+// values, where a level's time is its chunk's chain of sequential products, so
+// they use chunks of kInvChunkUpper.
+constexpr long kInvChunk0 = 32, kInvChunkUpper = 4;
+long inv_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, long count);
+inline int inv_chunk(size_t level, long chunk0) { return (int)(level == 0 ? chunk0 : kInvChunkUpper); }
 
 // Element counts per level of a batch inversion (up to one value).
-std::vector<long> inv_levels(long count);
+std::vector<long> inv_levels(long count, long chunk0);
 
 // Chunking of segments for a reduction level.  Level 0 reads the inputs in
 // chunks of up to kReduceChunk0 (kReduceChunkDot0 for dot terms, each a power),
