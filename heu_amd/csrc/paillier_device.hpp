@@ -1085,13 +1085,15 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_inv_down(Mod
 // ---------------------------------------------------------------------------
 // The one inversion per batch: x^-1 mod m by Bernstein-Yang divsteps (D. J.
 // Bernstein, B.-Y. Yang, "Fast constant-time gcd computation and modular
-// inversion", 2019), 60 divsteps at a time.  One wave holds f, g (the gcd pair,
+// inversion", 2019), 60 divsteps at a time.  Wave 1 holds f, g (the gcd pair,
 // starting at m and x) and d, e (f = d x, g = e x mod m) as 128 signed limbs of
 // 60 bits, two per lane.  A batch runs 60 divsteps on the low 64 bits of f and
-// g (uniform values, the same in every lane), which yield the transition matrix
-// [u v; q r] with |u| + |v|, |q| + |r| <= 2^60; then f, g <- [u v; q r] (f, g)
-// / 2^60 and d, e likewise after adding the multiple k m of m that makes them
-// divisible.  Limbs stay redundant, in [-2, 2^60 + 1] with the top one whole,
+// g (uniform values), which yield the transition matrix [u v; q r] with
+// |u| + |v|, |q| + |r| <= 2^60; then f, g <- [u v; q r] (f, g) / 2^60 and d, e
+// likewise after adding the multiple k m of m that makes them divisible.  Wave
+// 0 runs the divsteps one batch ahead: from the exact low 180 bits of f and g
+// before a batch and that batch's matrix it has the low bits after it, so the
+// next matrix is ready when wave 1 finishes applying the current one.  Limbs stay redundant, in [-2, 2^60 + 1] with the top one whole,
 // so they are the exact integers and no carry crosses more than one lane per
 // round; one lane resolves them to check for the end.  After the published
 // bound of (49 N + 57) / 17 divsteps (N = bits of m) g = 0 and f = +-gcd(x, m),
@@ -1114,6 +1116,13 @@ __device__ __forceinline__ i64 sg_first64(i64 x) {  // lane 0's value, in every 
 // The low 64 bits of the exact integer (limb 0 and the low bits of limb 1).
 __device__ __forceinline__ u64 sg_low64(const SgNum& x) {
   return (u64)sg_first64(x.l[0]) + ((u64)sg_first64(x.l[1]) << kSgBits);
+}
+
+// The exact low 128 bits of a number from its three lowest limbs, in every lane.
+__device__ __forceinline__ unsigned __int128 sg_low128(const i64* l3) {
+  typedef unsigned __int128 u128;
+  return (u128)(__int128)sg_first64(l3[0]) + ((u128)(__int128)sg_first64(l3[1]) << kSgBits) +
+         ((u128)(__int128)sg_first64(l3[2]) << (2 * kSgBits));
 }
 
 // 60 divsteps (delta, f, g) -> (1 - delta, g, (g - f) / 2) if delta > 0 and g
@@ -1215,7 +1224,7 @@ __device__ __forceinline__ i64 sg_resolve(i64* io) {
 // resolves the result, and the group maps it back (x R -> x^-1 R).  Sets
 // kErrNotInvertible if gcd(x, m) != 1.  One wave, launched once per batch.
 template <int L, int K, int W>
-__global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __restrict__ m_words, u32* xio, u32* err) {
+__global__ __launch_bounds__(128) void k_inv_single(ModArgs mod, const u32* __restrict__ m_words, u32* xio, u32* err) {
   constexpr int NL = L * K;
   static_assert(kSgBits * kSgLimbs >= 32 * W + 64, "room for the divsteps numbers");
   static_assert(29 * NL >= 32 * (W + 1), "room for the result before reduction");
@@ -1223,7 +1232,7 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
   __shared__ u32 words[W + 2];
   __shared__ i64 res[3][kSgLimbs];
   __shared__ int s_state;  // 0 running, 1 done (+2: negate), -1 not invertible
-  for (int i = threadIdx.x; i < NL; i += 64) lds[i] = mod.m[i];
+  for (int i = threadIdx.x; i < NL; i += 128) lds[i] = mod.m[i];
   __syncthreads();
   Grp<L, K, W> g;
   g.lid = grp_lane<L>();
@@ -1242,7 +1251,14 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
     for (int w = (int)g.lid; w < W; w += L) words[w] = limbs_to_word<L, K>(g.A, w);
   }
   __syncthreads();
-  const u32 lane = threadIdx.x;
+  // Wave 0 runs the divsteps, one batch ahead; wave 1 holds the numbers and
+  // applies each batch's matrix.  Per batch, wave 1 publishes the three lowest
+  // limbs of f and g (their exact low 180 bits), and wave 0 publishes the next
+  // batch's matrix, in buffers alternating by batch parity.
+  const u32 lane = threadIdx.x & 63u;
+  const bool steps_wave = threadIdx.x < 64u;
+  __shared__ i64 s_low[2][2][3];  // [parity][f, g][limb 0..2]
+  __shared__ i64 s_mat[2][4];     // [parity][u, v, q, r]
   SgNum f, gg, d, e;
   u64 ml[2];
 #pragma unroll
@@ -1253,6 +1269,17 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
     d.l[j] = 0;
     e.l[j] = (lane == 0u && j == 0) ? 1 : 0;
   }
+  auto publish_low = [&](int buf) {
+    if (lane == 0u) {
+      s_low[buf][0][0] = f.l[0];
+      s_low[buf][0][1] = f.l[1];
+      s_low[buf][1][0] = gg.l[0];
+      s_low[buf][1][1] = gg.l[1];
+    } else if (lane == 1u) {
+      s_low[buf][0][2] = f.l[0];
+      s_low[buf][1][2] = gg.l[0];
+    }
+  };
   // m^-1 mod 2^64 by Newton's iteration (m odd: m m = 1 mod 8 to start)
   const u64 m0 = (u64)m_words[0] | ((u64)m_words[1] << 32);
   u64 minv = m0;
@@ -1264,36 +1291,69 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
   // and once g = 0 further batches leave f and d unchanged: check from 70% of
   // the bound, every 4 batches up to it, every 8 after it.
   const int first = batches * 7 / 10;
-  int delta = 1, done = 0;
+  int delta = 1, done = 0, ph = 0;
+  i64 mu = 0, mv = 0, mq = 0, mr = 0;  // wave 0: the matrix of the batch wave 1 applies next
+  if (!steps_wave) publish_low(0);
+  __syncthreads();
+  if (steps_wave) {
+    delta = sg_divsteps(delta, (u64)sg_low128(s_low[0][0]), (u64)sg_low128(s_low[0][1]), mu, mv, mq, mr);
+    if (lane == 0u) {
+      s_mat[0][0] = mu;
+      s_mat[0][1] = mv;
+      s_mat[0][2] = mq;
+      s_mat[0][3] = mr;
+    }
+  }
+  __syncthreads();
   for (;;) {
     const int todo = done == 0 ? first : done < batches ? 4 : 8;
-    for (int bi = 0; bi < todo; ++bi) {
-      i64 u, v, q, r;
-      delta = sg_divsteps(delta, sg_low64(f), sg_low64(gg), u, v, q, r);
-      const u64 d0 = (u64)sg_first64(d.l[0]), e0 = (u64)sg_first64(e.l[0]);
-      const u64 kd = (0u - ((u64)u * d0 + (u64)v * e0) * minv) & kSgMask;
-      const u64 ke = (0u - ((u64)q * d0 + (u64)r * e0) * minv) & kSgMask;
-      const u64 none[2] = {0u, 0u};
-      SgNum nf, ng, nd, ne;
-      sg_lincomb(nf, u, v, f, gg, 0u, none, lane);
-      sg_lincomb(ng, q, r, f, gg, 0u, none, lane);
-      sg_lincomb(nd, u, v, d, e, kd, ml, lane);
-      sg_lincomb(ne, q, r, d, e, ke, ml, lane);
-      f = nf;
-      gg = ng;
-      d = nd;
-      e = ne;
+    for (int bi = 0; bi < todo; ++bi, ++ph) {
+      const int cur = ph & 1, nxt = cur ^ 1;
+      if (steps_wave) {
+        // low bits of the numbers after this batch, then the next batch's matrix
+        typedef unsigned __int128 u128;
+        const u128 F = sg_low128(s_low[cur][0]), G = sg_low128(s_low[cur][1]);
+        const u128 F1 = (u128)(__int128)mu * F + (u128)(__int128)mv * G;
+        const u128 G1 = (u128)(__int128)mq * F + (u128)(__int128)mr * G;
+        delta = sg_divsteps(delta, (u64)(F1 >> kSgBits), (u64)(G1 >> kSgBits), mu, mv, mq, mr);
+        if (lane == 0u) {
+          s_mat[nxt][0] = mu;
+          s_mat[nxt][1] = mv;
+          s_mat[nxt][2] = mq;
+          s_mat[nxt][3] = mr;
+        }
+      } else {
+        const i64 u = sg_first64(s_mat[cur][0]), v = sg_first64(s_mat[cur][1]);
+        const i64 q = sg_first64(s_mat[cur][2]), r = sg_first64(s_mat[cur][3]);
+        const u64 d0 = (u64)sg_first64(d.l[0]), e0 = (u64)sg_first64(e.l[0]);
+        const u64 kd = (0u - ((u64)u * d0 + (u64)v * e0) * minv) & kSgMask;
+        const u64 ke = (0u - ((u64)q * d0 + (u64)r * e0) * minv) & kSgMask;
+        const u64 none[2] = {0u, 0u};
+        SgNum nf, ng, nd, ne;
+        sg_lincomb(nf, u, v, f, gg, 0u, none, lane);
+        sg_lincomb(ng, q, r, f, gg, 0u, none, lane);
+        sg_lincomb(nd, u, v, d, e, kd, ml, lane);
+        sg_lincomb(ne, q, r, d, e, ke, ml, lane);
+        f = nf;
+        gg = ng;
+        d = nd;
+        e = ne;
+        publish_low(nxt);
+      }
+      __syncthreads();
     }
     done += todo;
-    // resolve g (then f and d) in lane 0
+    // resolve g (then f and d) in thread 0
+    if (!steps_wave) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      res[0][2 * lane + j] = f.l[j];
-      res[1][2 * lane + j] = gg.l[j];
-      res[2][2 * lane + j] = d.l[j];
+      for (int j = 0; j < 2; ++j) {
+        res[0][2 * lane + j] = f.l[j];
+        res[1][2 * lane + j] = gg.l[j];
+        res[2][2 * lane + j] = d.l[j];
+      }
     }
     __syncthreads();
-    if (lane == 0u) {
+    if (threadIdx.x == 0u) {
       i64 *rf = res[0], *rg = res[1], *rd = res[2];
       bool g_zero = sg_resolve(rg) == 0;
       for (int i = 0; i < kSgLimbs && g_zero; ++i) g_zero = rg[i] == 0;
@@ -1335,7 +1395,7 @@ __global__ __launch_bounds__(64) void k_inv_single(ModArgs mod, const u32* __res
     if (s_state != 0) break;  // uniform
   }
   if (s_state < 0) {
-    if (lane == 0u) atomicOr(err, kErrNotInvertible);
+    if (threadIdx.x == 0u) atomicOr(err, kErrNotInvertible);
     return;  // uniform
   }
   const bool neg = (s_state & 2) != 0;
@@ -1565,7 +1625,7 @@ struct Launch {
                          nsel);
   }
   static void inv_single(hipStream_t st, const ModArgs& mod, const u32* m_words, u32* xio, u32* err) {
-    hipLaunchKernelGGL((k_inv_single<L, K, W>), dim3(1), dim3(64), 0, st, mod, m_words, xio, err);
+    hipLaunchKernelGGL((k_inv_single<L, K, W>), dim3(1), dim3(128), 0, st, mod, m_words, xio, err);
   }
   static void mul_const(hipStream_t st, const ModArgs& mod, const u32* in, const u32* cz, long n, u32* out) {
     HEU_LAUNCH((k_mul_const<L, K, W>), n, L, st, mod, in, cz, n, out);
