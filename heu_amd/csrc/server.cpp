@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -65,6 +66,7 @@ struct Request {
   bool whole = false;                             // SUM / DOT: one part
   long whole_units = 0;
   long n = 0, admitted = 0, finished = 0;         // elements (1 for a whole request)
+  long inflight = 0;                              // parts admitted to a slot and not yet retired
   size_t sa = 0, sb = 0, so = 0;                  // bytes per element of a, b and out
   Clock::time_point t_submit, t_done;
   int state = kQueued;
@@ -123,6 +125,7 @@ struct Slot {
   std::vector<std::pair<uint64_t, long>> parts;  // (ticket, elements)
   char* pinned = nullptr;                         // staging of the batch's work lists
   size_t pinned_cap = 0;
+  bool drained = false;                           // the stream was synchronized: retire without the event
 };
 
 // First-fit allocator over [0, size) with coalescing.
@@ -191,11 +194,19 @@ struct heu_server {
   std::thread worker;
   bool stop = false;
   heu_server_counters st{};
+  // Failure injection for tests (HEU_SERVER_FAULT_BATCH=k): the k-th batch is
+  // reported as not enqueued after its launches were issued, so its requests
+  // fail while their kernels still run.
+  uint64_t fault_batch = 0;
 
   void run();
   bool retire();
   bool admit(std::unique_lock<std::mutex>& lk);
   bool select(std::vector<Pick>* picks, const heu_hip::WorklistShape** shape);
+  // A failed request is reported (and forgotten by heu_server_wait) only once
+  // none of its parts is still on the device: until then the caller's buffers
+  // may still be read or written by a running batch.
+  static bool settled(const Request& r) { return r.state == kDone || (r.state == kFailed && r.inflight == 0); }
   void fail_request(uint64_t t, const std::string& why) {
     Request& r = reqs[t];
     if (r.state == kFailed) return;
@@ -211,13 +222,15 @@ struct heu_server {
 bool heu_server::retire() {
   bool any = false;
   for (Slot& s : slots) {
-    if (!s.busy || hipEventQuery(s.done) != hipSuccess) continue;
+    if (!s.busy || (!s.drained && hipEventQuery(s.done) != hipSuccess)) continue;
     std::lock_guard<std::mutex> lk(mu);
     const auto now = Clock::now();
     for (const auto& pt : s.parts) {
       auto it = reqs.find(pt.first);
-      if (it == reqs.end() || it->second.state == kFailed) continue;
+      if (it == reqs.end()) continue;
       Request& r = it->second;
+      --r.inflight;
+      if (r.state == kFailed) continue;
       r.finished += pt.second;
       if (r.finished == r.n) {
         r.state = kDone;
@@ -228,6 +241,7 @@ bool heu_server::retire() {
     arena.give(s.off, s.bytes);
     in_use -= s.bytes;
     s.busy = false;
+    s.drained = false;
     s.parts.clear();
     any = true;
   }
@@ -388,6 +402,7 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
     r.state = kRunning;
     units += r.units_for(picks[i].count);
     r.admitted += picks[i].count;
+    ++r.inflight;
     slot->parts.emplace_back(picks[i].ticket, picks[i].count);
     if (r.admitted == r.n) pending.erase(std::find(pending.begin(), pending.end(), picks[i].ticket));
   }
@@ -396,7 +411,7 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   slot->units = units;
   slot->off = off;
   slot->bytes = bytes;
-  ++st.batches;
+  const uint64_t batch_no = ++st.batches;
   st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, k);
   lk.unlock();
   PlanOut real;
@@ -427,15 +442,22 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
     (void)hipGetLastError();  // only this batch's launches count below
     for (auto& s : real.steps) s(slot->stream);
     if (hipGetLastError() != hipSuccess) rc = HEU_EDEVICE;
+    if (fault_batch && batch_no == fault_batch) rc = HEU_EDEVICE;
   }
-  if (hipEventRecord(slot->done, slot->stream) != hipSuccess) rc = HEU_EDEVICE;
+  const bool recorded = hipEventRecord(slot->done, slot->stream) == hipSuccess;
+  if (!recorded) {
+    // without the event, retire() cannot see when the device is done with the
+    // parts: wait for the stream here instead
+    rc = HEU_EDEVICE;
+    (void)hipStreamSynchronize(slot->stream);
+  }
   lk.lock();
   if (rc != HEU_OK) {
-    // nothing (or not all) was enqueued: the requests fail; the slot retires
-    // through its event as usual
+    // nothing (or not all) was enqueued: the requests fail.  The slot keeps
+    // its parts and retires as usual, which releases each part's in-flight
+    // count; heu_server_wait reports a failed request only after that.
     for (const auto& pt : slot->parts) fail_request(pt.first, "could not enqueue the batch");
-    slot->parts.clear();
-    cv_done.notify_all();
+    slot->drained = !recorded;
   }
   return true;
 }
@@ -484,6 +506,7 @@ int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** o
   if (!srv->cfg.parts_per_round) srv->cfg.parts_per_round = 8;
   if (!srv->cfg.urgent_ms) srv->cfg.urgent_ms = 300;
   if (srv->cfg.max_inflight > 8) return fail(HEU_EINVAL, "max_inflight must be at most 8");
+  if (const char* f = std::getenv("HEU_SERVER_FAULT_BATCH")) srv->fault_batch = std::strtoull(f, nullptr, 10);
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = srv->ws.alloc(round256(srv->cfg.workspace_bytes));
   if (rc) return rc;
@@ -542,7 +565,7 @@ int heu_server_wait(heu_server* srv, uint64_t ticket, double timeout_ms, double*
   if (it == srv->reqs.end()) return fail(HEU_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
   auto ready = [&] {
     auto i = srv->reqs.find(ticket);
-    return i == srv->reqs.end() || i->second.state == kDone || i->second.state == kFailed;
+    return i == srv->reqs.end() || heu_server::settled(i->second);
   };
   if (timeout_ms < 0) srv->cv_done.wait(lk, ready);
   else if (!srv->cv_done.wait_for(lk, std::chrono::microseconds((long long)(timeout_ms * 1e3)), ready))

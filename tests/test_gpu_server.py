@@ -172,3 +172,41 @@ def test_long_request_does_not_block_short_ones(eng, ctx, keys):
     for i, (_, out) in enumerate(subs):
         assert np.array_equal(host(out).view(np.int64).ravel(), np.array([i, -i, 7 * i], np.int64))
     srv.close()
+
+
+def test_failed_part_waits_for_running_parts(eng, ctx, keys, monkeypatch):
+    # One long decryption is carved into parts of one quantum each
+    # (max_batch_tasks=1), so several of its parts run at once on different
+    # streams.  The second batch is reported as not enqueued after its
+    # launches were issued (HEU_SERVER_FAULT_BATCH, forced on the host; every
+    # kernel still runs to its end).  The request fails, but wait() may report
+    # that only once no part of it is still on the device: by then every
+    # admitted part has written its decryptions, and the rest of the output was
+    # never touched.
+    key = keys[0]
+    n = 120000
+    m = (np.arange(n, dtype=np.int64) * 7919) - n
+    ct = key.encrypt_i64(m, eng.seeded_r(12, 1, n, 256))
+    ctx.sync()
+    sentinel = -0x5A5A5A5A
+    monkeypatch.setenv("HEU_SERVER_FAULT_BATCH", "2")
+    srv = eng.Server(ctx, max_inflight=4, max_batch_tasks=1)
+    monkeypatch.delenv("HEU_SERVER_FAULT_BATCH")
+    out = torch.full((n, 2), sentinel, dtype=torch.int32, device=ctx.device)
+    torch.cuda.synchronize()  # the server's streams do not wait for torch's
+    t, _ = srv._record(key, eng._native.HEU_OP_DECRYPT, key._dev(ct, torch.int32), None, n, out, aux=2)
+    from heu_amd._native import HeuError
+
+    print("server: one carved decryption, batch 2 fails", flush=True)
+    with pytest.raises(HeuError, match="could not enqueue"):
+        srv.wait(t, 120000.0)
+    got = out.cpu().numpy()  # read at once: no device-wide sync in between
+    done = ~np.all(got == sentinel, axis=1)
+    x = int(np.argmin(done)) if not done.all() else n
+    print(f"decrypted prefix {x} of {n}", flush=True)
+    assert 0 < x < n
+    assert done[:x].all() and not done[x:].any()
+    assert np.array_equal(got[:x].view(np.int64).ravel(), m[:x])
+    st = srv.stats()
+    assert st["failed"] == 1 and st["batches"] >= 2
+    srv.close()
