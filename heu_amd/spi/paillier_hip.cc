@@ -242,11 +242,12 @@ struct Runtime {
 
   char* reserve(size_t bytes) {
     if (bytes > cap) {
+      // grow geometrically, so a run of slightly larger calls reallocates rarely
+      const size_t want = std::max(bytes, cap * 2);
       hip_ck(hipStreamSynchronize(stream), "hipStreamSynchronize");
       if (arena) hip_ck(hipFree(arena), "hipFree");
       arena = nullptr;
       cap = 0;
-      size_t want = std::max(bytes, cap * 2);
       hip_ck(hipMalloc(reinterpret_cast<void**>(&arena), want), "hipMalloc");
       cap = want;
     }
