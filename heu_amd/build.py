@@ -21,7 +21,7 @@ ARCH = os.environ.get("HEU_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["ops_2048.hip", "ops_3072.hip", "worklist_2048.hip", "worklist_3072.hip", "worklist_dec_2048.hip",
            "worklist_dec_3072.hip", "engine.cpp", "batch.cpp", "server.cpp", "tune.cpp",
-           "keygen.cpp"]  # valu_peak.hip builds separately
+           "keygen.cpp"]
 # Packed FP32 instructions stay off in device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -60,14 +60,6 @@ def build(force: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stderr}")
     os.replace(tmp, LIB)
-    # the VALU issue-rate probe (a standalone executable)
-    probe = os.path.join(HERE, "valu_peak")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", *DEVICE_FLAGS, "-o", probe + ".tmp",
-           os.path.join(CSRC, "valu_peak.hip")]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{res.stderr}")
-    os.replace(probe + ".tmp", probe)
     build_spi()
     return LIB
 
