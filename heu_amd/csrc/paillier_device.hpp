@@ -635,11 +635,18 @@ __device__ __forceinline__ void mont32(u32 (&r)[PW], const u32 (&a)[PW], const u
   for (int j = 0; j < PW; ++j) r[j] = ge ? d[j] : t[j];
 }
 
+// For a ciphertext coprime to p, u = c^(p-1) mod p^2 is 1 mod p and the
+// division is exact.  A value sharing the factor p (not a ciphertext) gives
+// u = 0; there (u - 1) / p truncates to 0, as the big-integer division of the
+// restated decryptor does, so L = 0.
 template <int PW>
 __device__ __forceinline__ void l_times_h(u32 (&out)[PW], const u32* __restrict__ u, const u32* __restrict__ pinv,
                                           const u32* __restrict__ hR, const u32* __restrict__ p, u32 pp) {
+  u32 nz = 0;
+#pragma unroll
+  for (int j = 0; j < 2 * PW; ++j) nz |= u[j];
   u32 um1[PW];
-  i64 br = -1;
+  i64 br = nz ? -1 : 0;
 #pragma unroll
   for (int j = 0; j < PW; ++j) {
     i64 v = (i64)u[j] + br;
