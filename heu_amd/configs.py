@@ -17,6 +17,9 @@ keyed by (config id << 32 | seed, stream); the streams are listed per config.
 
 Each runner times the plain path (one engine call per op and vector, one op
 per launch) beside the batched form and checks that both give the same bytes.
+Config 5 also replays its arrivals through the plain batched path (static
+batching: whatever has arrived runs as one batch once the device is idle),
+whose per-batch time is what served tail latency is measured against.
 """
 from __future__ import annotations
 
@@ -63,6 +66,13 @@ def chain_expected(m: np.ndarray, m2: np.ndarray, k: np.ndarray) -> np.ndarray:
     vals = ((int(a) + int(b)) * int(c) for a, b, c in zip(m.tolist(), m2.tolist(), k.tolist()))
     raw = b"".join(v.to_bytes(16, "little", signed=True) for v in vals)
     return np.frombuffer(raw, dtype=np.uint32).reshape(len(m), 4)
+
+
+def dot_closed_form(m: np.ndarray, w: np.ndarray, offs: np.ndarray, i: int) -> int:
+    """sum m*w over segment i as a Python integer (exact: int64 products in
+    object arithmetic)."""
+    a, e = int(offs[i]), int(offs[i + 1])
+    return int(np.dot(m[a:e].astype(object), w[a:e].astype(object))) if e > a else 0
 
 
 def _dev(a, dtype=None):
@@ -120,14 +130,10 @@ def run_config3(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
     st = b.stats()
     batch_ms = min(_elapsed_ms(b.run, ctx) for _ in range(reps))
     same = all(torch.equal(dots[i:i + 1], plain_out[i]) for i in range(64))
-    # closed form on a few vectors: Dec(dot) = sum m*w
-    dec = key.decrypt(dots[:8], out_words=4).cpu().numpy().view(np.uint32)
+    # closed form on every vector: Dec(dot) = sum m*w
+    dec = key.decrypt(dots, out_words=4).cpu().numpy().view(np.uint32)
     m_h, w_h = m.cpu().numpy(), w.cpu().numpy()
-    closed = True
-    for i in range(8):
-        a, e = int(offs[i]), int(offs[i + 1])
-        want = sum(int(x) * int(y) for x, y in zip(m_h[a:e], w_h[a:e]))
-        closed &= E.twos_to_int(dec[i]) == want
+    closed = all(E.twos_to_int(dec[i]) == dot_closed_form(m_h, w_h, offs, i) for i in range(64))
     b.close()
     key.close()
     return {
@@ -143,6 +149,7 @@ def run_config3(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
         "bit_exact_vs_plain": bool(same),
         "closed_form_check": bool(closed),
         "autotune": tune,
+        "work": {"enc": {3072: total}, "dot": {3072: total}},
     }
 
 
@@ -167,6 +174,7 @@ def run_config4(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
     lens = log_uniform_lengths(4, seed, 1, 16, 256, 262144)
     jobs = []  # (op, key, inputs)
     total = 0
+    work = {op: {2048: 0, 3072: 0} for op in ("enc", "dec", "add", "mul", "sum")}
     for i, (key, n) in enumerate(zip(keys, lens)):
         n = int(n)
         cut = [0, int(0.4 * n), int(0.6 * n), int(0.8 * n), n]
@@ -180,11 +188,14 @@ def run_config4(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
         jobs.append(("dec", key, (ct[a1:a2],)))
         jobs.append(("add", key, (ct[a2:a3], ct2[a2:a3])))
         jobs.append(("mul", key, (ct[a3:a4], k[a3:a4])))
+        for op, lo, hi in (("enc", a0, a1), ("dec", a1, a2), ("add", a2, a3), ("mul", a3, a4)):
+            work[op][key.key_bits] += hi - lo
         total += n
     # HeMatrix 4096 x 256 (column-major) axis-0 sum on key 0
     rows, cols = 4096, 256
     mat = keys[0].encrypt_i64(_dev(int64s(4, seed, 400, rows * cols) >> 16), _dev(r_limbs(4, seed, 401, rows * cols).view(np.int32)))
     col_offs = np.arange(0, rows * cols + 1, rows, dtype=np.int64)
+    work["sum"][2048] += rows * cols
     ctx.sync()
 
     def plain():
@@ -238,6 +249,7 @@ def run_config4(ctx: E.Context, seed: int = 0, reps: int = 3) -> dict:
         "work_units": st["units"],
         "bit_exact_vs_plain": bool(same),
         "autotune": tune,
+        "work": work,
     }
 
 
@@ -273,7 +285,7 @@ def _requests5(seed, count, nkeys):
 def _call(target, pool: _Pool, op, n):
     """Runs / submits one request through `target` (a Key, or a Server taking the key first)."""
     key = pool.key
-    server = isinstance(target, E.Server)
+    server = isinstance(target, (E.Server, E.Batch))
     args = {
         "enc": (pool.m[:n], pool.r[:n]),
         "dec": (pool.ct[:n],),
@@ -300,6 +312,39 @@ def _plain_fifo(ctx, pools, rq, arrivals):
         ctx.sync()
         lat.append((time.perf_counter() - t0 - arrivals[j]) * 1e3)
     return lat, time.perf_counter() - t0
+
+
+def _plain_batched(ctx, pools, rq, arrivals):
+    """The plain batched path (static batching): whenever the device is idle,
+    every request that has arrived is planned into one batch and run to
+    completion; requests arriving meanwhile wait for the next batch.  Returns
+    latencies (completion - arrival) and each batch's time (plan + run)."""
+    lat = [0.0] * len(rq)
+    batch_ms = []
+    t0 = time.perf_counter()
+    j = 0
+    while j < len(rq):
+        wait = arrivals[j] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        now = time.perf_counter() - t0
+        k = j
+        while k < len(rq) and arrivals[k] <= now:
+            k += 1
+        tb = time.perf_counter()
+        b = E.Batch(ctx, graph=False)
+        for jj in range(j, k):
+            op, ki, n = rq[jj]
+            _call(b, pools[ki], op, n)
+        b.run()
+        ctx.sync()
+        done = time.perf_counter()
+        b.close()
+        batch_ms.append((done - tb) * 1e3)
+        for jj in range(j, k):
+            lat[jj] = (done - t0 - arrivals[jj]) * 1e3
+        j = k
+    return lat, time.perf_counter() - t0, batch_ms
 
 
 def _served(srv, pools, rq, arrivals):
@@ -337,7 +382,8 @@ def _summary(rate, lat, wall):
     }
 
 
-def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0), per_rate: int = 320) -> dict:
+def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0, 60.0, 90.0, 120.0),
+                per_rate: int = 320) -> dict:
     pools = []
     for i in range(32):
         bits = 2048 if i < 24 else 3072
@@ -345,16 +391,31 @@ def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0), per_rat
         pools.append(_Pool(key, seed, i, MAX_LEN5))
     ctx.sync()
     srv = E.Server(ctx, workspace_bytes=24 << 30, max_inflight=2)
-    plain, served = [], []
+    plain, batched, served = [], [], []
+    work = {op: {2048: 0, 3072: 0} for op in OPS5}
     check_same = None
     for ri, rate in enumerate(rates):
         rq = _requests5(seed + 1 + ri, per_rate, len(pools))
+        for op, ki, n in rq:
+            work[op][pools[ki].key.key_bits] += n
         gaps = -np.log(1.0 - uniform01(5, seed + 1 + ri, 20, per_rate)) / rate
         arrivals = np.cumsum(gaps)
+        print(f"[config5] {rate:g} req/s: plain FIFO", flush=True)
         lat_p, wall_p = _plain_fifo(ctx, pools, rq, arrivals)
         plain.append(_summary(rate, lat_p, wall_p))
+        print(f"[config5] {rate:g} req/s: plain batched", flush=True)
+        lat_b, wall_b, batch_ms = _plain_batched(ctx, pools, rq, arrivals)
+        row = _summary(rate, lat_b, wall_b)
+        row.update({"batches": len(batch_ms), "batch_ms_mean": float(np.mean(batch_ms)),
+                    "batch_ms_p50": float(np.percentile(batch_ms, 50)),
+                    "batch_ms_p99": float(np.percentile(batch_ms, 99))})
+        batched.append(row)
+        print(f"[config5] {rate:g} req/s: served", flush=True)
         lat_s, wall_s, tickets = _served(srv, pools, rq, arrivals)
-        served.append(_summary(rate, lat_s, wall_s))
+        row = _summary(rate, lat_s, wall_s)
+        # the north star's bar: served p99 <= 2x the plain batched path's per-batch time
+        row["p99_over_2x_plain_batch_mean"] = row["p99_ms"] / (2.0 * batched[-1]["batch_ms_mean"])
+        served.append(row)
         if ri == 0:
             # alone vs in the mix: these requests again, one engine call each
             same = True
@@ -371,7 +432,10 @@ def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0), per_rat
         "workload": "32 resident keys (24x2048, 8x3072); Poisson arrivals; enc/dec/add/mul/dot 30/30/15/15/10; "
                     "lengths log-uniform 1..262144",
         "plain_fifo": plain,
+        "plain_batched": batched,
         "served": served,
+        "work": work,
+        "requests_in_sweep": per_rate * len(rates),
         "server_batches": st["batches"],
         "max_requests_per_batch": st["max_batch_requests"],
         "alone_equals_mix": check_same,
