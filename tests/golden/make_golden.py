@@ -9,6 +9,7 @@ decryptor.cc:32-53, evaluator.cc:25-123):
   Dec(c)   = L(c^lambda mod n^2) * mu mod n, folded to (-n/2, n/2]
   Add      = c1*c2 mod n^2;  AddPt = c*(1+n*p) mod n^2
   Mul(c,k) = c^k mod n^2 (k < 0: (c^-1)^|k|);  Negate = c^-1 mod n^2
+  Sub      = c1 * c2^-1 mod n^2;  Randomize(c, r3) = c * h_s^r3 mod n^2
 The keys come from the oracle's seeded key generator (key material only);
 their invariants are re-checked here with Python integers.
 
@@ -74,7 +75,7 @@ def key_entry(bits, seed):
     return kd
 
 
-def vectors(kd, rng, count):
+def vectors(kd, rng, count, rng_extra):
     n, hs, p, q = kd["n"], kd["hs"], kd["p"], kd["q"]
     n2 = n * n
     out = []
@@ -89,6 +90,7 @@ def vectors(kd, rng, count):
         c2 = encrypt(m2, r2, n, hs)
         assert decrypt(c, n, p, q) == m
         mul = pow(c, k, n2) if k >= 0 else pow(pow(c, -1, n2), -k, n2)
+        r3 = rng_extra.randrange(1, 2**256)
         out.append({
             "m": str(m), "r": hex(r), "c": hex(c),
             "m2": str(m2), "r2": hex(r2), "c2": hex(c2),
@@ -96,6 +98,8 @@ def vectors(kd, rng, count):
             "add_pt": hex(c * ((1 + n * m2) % n2) % n2),
             "k": str(k), "mul": hex(mul),
             "negate": hex(pow(c, -1, n2)),
+            "sub": hex(c * pow(c2, -1, n2) % n2),
+            "r3": hex(r3), "randomize": hex(c * pow(hs, r3, n2) % n2),
         })
     return out
 
@@ -108,7 +112,7 @@ def main():
         data["keys"].append({
             "bits": bits, "seed": seed,
             "n": hex(kd["n"]), "p": hex(kd["p"]), "q": hex(kd["q"]), "hs": hex(kd["hs"]),
-            "vectors": vectors(kd, rng, count),
+            "vectors": vectors(kd, rng, count, random.Random(1000 + bits)),
         })
     with open(os.path.join(HERE, "paillier_vectors.json"), "w") as f:
         json.dump(data, f, indent=1)

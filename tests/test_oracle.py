@@ -44,6 +44,12 @@ def test_golden_vectors(oracle_mod, golden):
         ks = np.array([int(v["k"]) for v in vs], np.int64)
         assert O.rows_to_ints(key.mul_i64(ct, ks)) == [int(v["mul"], 16) for v in vs]
         assert O.rows_to_ints(key.negate(ct)) == [int(v["negate"], 16) for v in vs]
+        # Sub (evaluator.cc:66-68) and Randomize (evaluator.cc:25-28), directly
+        assert O.rows_to_ints(key.sub(ct, c2)) == [int(v["sub"], 16) for v in vs]
+        r3 = O.ints_to_rows([int(v["r3"], 16) for v in vs], 8)
+        assert O.rows_to_ints(key.randomize(ct, r3)) == [int(v["randomize"], 16) for v in vs]
+        key.build_table(10, 256)  # the fixed-base table route gives the same values
+        assert O.rows_to_ints(key.randomize(ct, r3, use_table=True)) == [int(v["randomize"], 16) for v in vs]
         # int64 fast path agrees where m fits
         fit = [i for i, x in enumerate(m) if -(2**63) <= x < 2**63]
         ct64 = key.encrypt_i64(np.array([m[i] for i in fit], np.int64), r[fit])
