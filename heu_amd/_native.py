@@ -41,6 +41,7 @@ SIGNATURES = {
     "heu_mul_i64": (I, [P, P, P, SZ, P]),
     "heu_mul": (I, [P, P, P, P, U32, SZ, P]),
     "heu_sum": (I, [P, P, P, SZ, P]),
+    "heu_sum_noalloc": (I, [P, P, P, SZ, P]),
     "heu_dot_i64": (I, [P, P, P, P, SZ, P]),
     "heu_sum_gather": (I, [P, P, P, P, SZ, P]),
     "heu_dot_gather": (I, [P, P, P, P, P, SZ, P]),

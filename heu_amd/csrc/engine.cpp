@@ -287,10 +287,15 @@ int build_table(heu_key* key, uint32_t window_bits) {
 // Carves 256-aligned parts from the context's scratch pool, starting at
 // `base`.  The pool is reallocated (after draining the stream) only by a
 // top-level carve (base == 0); nested carves must fit what the caller reserved.
+// Over a fixed buffer (`fixed`, `fixed_bytes`) nothing is ever allocated: a
+// carve that does not fit fails.
 struct Scratch {
   heu_ctx* ctx;
   size_t base, total = 0;
+  char* fixed = nullptr;
+  size_t fixed_bytes = 0;
   explicit Scratch(heu_ctx* c, size_t b = 0) : ctx(c), base(b) {}
+  Scratch(heu_ctx* c, char* buf, size_t bytes) : ctx(c), base(0), fixed(buf), fixed_bytes(bytes) {}
   size_t add(size_t bytes) {
     size_t off = base + total;
     total += (bytes + 255) & ~(size_t)255;
@@ -298,6 +303,12 @@ struct Scratch {
   }
   size_t end() const { return base + total; }
   int commit() {
+    if (fixed) {
+      if (end() > fixed_bytes)
+        return fail(HEU_ERANGE, "the call needs %zu bytes of scratch; the fixed buffer holds %zu", end(),
+                    fixed_bytes);
+      return HEU_OK;
+    }
     if (end() > ctx->scratch.bytes) {
       if (base != 0) return fail(HEU_EINVAL, "internal: nested scratch exceeds the reservation");
       HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -308,7 +319,7 @@ struct Scratch {
   }
   template <class T>
   T* at(size_t off) const {
-    return reinterpret_cast<T*>(static_cast<char*>(ctx->scratch.p) + off);
+    return reinterpret_cast<T*>((fixed ? fixed : static_cast<char*>(ctx->scratch.p)) + off);
   }
 };
 
@@ -392,14 +403,16 @@ int upload_meta(heu_ctx* ctx, Scratch& sc, size_t off, const void* host, size_t 
 }
 
 // Generic segmented reduction.  dot: weights given (device), else plain product.
+// With `fixed` set, a plain product carves its scratch from that buffer only.
 int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t* offsets, size_t segments,
-              uint32_t* out, const int64_t* idx = nullptr) {
+              uint32_t* out, const int64_t* idx = nullptr, char* fixed = nullptr, size_t fixed_bytes = 0) {
   heu_ctx* ctx = key->ctx;
   const Ops* o = key->ops;
   const ModArgs mod = key->mod_n2();
   const size_t NLB = (size_t)key->nl_n * 4;
   const bool dot = w != nullptr;
   if (segments == 0) return HEU_OK;
+  if (fixed && dot) return fail(HEU_EINVAL, "internal: a dot product needs the context's scratch pool");
   std::vector<long> start(segments), len(segments);
   for (size_t s = 0; s < segments; ++s) {
     start[s] = offsets[s];
@@ -431,7 +444,7 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
       lv.push_back(std::move(n));
     }
   }
-  Scratch sc(ctx);
+  Scratch sc = fixed ? Scratch(ctx, fixed, fixed_bytes) : Scratch(ctx);
   for (auto& l : lv) {
     l.ch_off = sc.add(l.ch.size() * sizeof(Chunk));
     l.pos_off = sc.add(l.ch.size() * NLB);
@@ -498,6 +511,9 @@ int heu_ctx_create(int device, heu_ctx** out) {
   HIP_TRY(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(HEU_EINVAL, "device %d not present (%d devices)", device, n);
   HIP_TRY(hipSetDevice(device));
+  // the device's fixed scratch of heu_sum_noalloc; a failure is kept for that
+  // entry point to report and does not stop the context
+  (void)noalloc_setup(device);
   auto ctx = new heu_ctx();
   ctx->device = device;
   hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
@@ -1011,3 +1027,15 @@ int heu_from_mont(heu_key* key, const uint32_t* ct, size_t count, uint32_t word_
 }
 
 }  // extern "C"
+
+namespace heu_engine {
+
+int sum_fixed_scratch(heu_key* key, const uint32_t* ct, const int64_t* offsets, size_t segments, uint32_t* out,
+                      char* buf, size_t bytes) {
+  KEY_PROLOGUE(key);
+  if (!offsets || (segments && !out) || (!ct && segments && offsets[segments] > offsets[0]))
+    return fail(HEU_EINVAL, "null argument");
+  return segmented(key, ct, nullptr, offsets, segments, out, nullptr, buf, bytes);
+}
+
+}  // namespace heu_engine

@@ -109,6 +109,16 @@ void ctx_release(heu_ctx* ctx);
 
 // Checks the tasks and plans them (caller holds ctx->mu).
 int plan_batch(heu_ctx* ctx, const heu_task* tasks, size_t ntasks, char* base, PlanOut* out);
+// heu_sum's segment product with every scratch part carved from the caller's
+// fixed device buffer [buf, buf + bytes) instead of the context's pool; fails
+// with HEU_ERANGE when the call needs more (engine.cpp).
+int sum_fixed_scratch(heu_key* key, const uint32_t* ct, const int64_t* offsets, size_t segments, uint32_t* out,
+                      char* buf, size_t bytes);
+// Looks up this device's address of heu_sum_noalloc's fixed scratch array,
+// once per device, with the device current (noalloc.hip); returns the HIP
+// status, which heu_sum_noalloc reports if the lookup failed.
+hipError_t noalloc_setup(int device);
+
 // Work-list units of `shape` that run at once on the whole device: resident
 // workgroups per CU x CUs.  A grid multiplier above 1 launches more workgroups
 // than that, but they queue; it does not change what runs at once (caller

@@ -396,6 +396,17 @@ class Key:
         check(lib().heu_sum(self._h, _ptr(ct), _ptr(offs), segs, _ptr(out)))
         return out
 
+    def sum_noalloc(self, ct, offsets, out: torch.Tensor) -> torch.Tensor:
+        """sum() into a caller-provided output, with no device allocation by the
+        engine (heu_sum_noalloc: scratch in a fixed per-GPU array)."""
+        ct = self._dev(ct, torch.int32)
+        offs = self._segments(offsets, self._rows(ct))
+        segs = len(offs) - 1
+        if not out.is_contiguous() or tuple(out.shape) != (segs, self.ct_words) or out.dtype != torch.int32:
+            raise ValueError(f"out must be a contiguous int32 tensor of shape ({segs}, {self.ct_words})")
+        check(lib().heu_sum_noalloc(self._h, _ptr(ct), _ptr(offs), segs, _ptr(out)))
+        return out
+
     def dot_i64(self, ct, w, offsets) -> torch.Tensor:
         ct, w = self._dev(ct, torch.int32), self._dev(w, torch.int64)
         offs = self._segments(offsets, self._rows(ct))

@@ -168,6 +168,20 @@ int heu_sum_gather(heu_key *key, const uint32_t *ct, const int64_t *idx, const i
                    uint32_t *out);
 int heu_dot_gather(heu_key *key, const uint32_t *ct, const int64_t *idx, const int64_t *w, const int64_t *offsets,
                    size_t segments, uint32_t *out);
+/* heu_sum for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, kernels and results as
+ * heu_sum; its scratch (chunk lists, the partial products of every reduction
+ * level) lives in a fixed array of HEU_NOALLOC_SCRATCH_BYTES in each GPU's
+ * memory, which that GPU holds from the first context created on it until the
+ * process ends, so the call makes no device allocation.  A call that needs
+ * more scratch fails with HEU_ERANGE before any launch (a 4096 x 256 matrix
+ * summed along axis 0 needs about 40 MiB with 3072-bit keys).  Every call on
+ * a GPU shares its array and the array keeps the last call's contents: calls
+ * on one context run in stream order, but calls that overlap on one GPU (other
+ * contexts or streams) race. */
+#define HEU_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int heu_sum_noalloc(heu_key *key, const uint32_t *ct, const int64_t *offsets, size_t segments, uint32_t *out);
 /* Inclusive prefix products within segments (the cumsum of
  * numpy/evaluator.cc:450-456): out[j] = ct[offsets[s]] * ... * ct[j] mod n^2
  * for j in segment s; out may alias ct. */
