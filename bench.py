@@ -47,8 +47,10 @@ def M(s: int) -> int:
 
 
 # Algorithmic multiply-adds per element (2048-bit keys: n^2 = 128 words, p^2 = 64 words).
+# Encryption's entry is set from the key's table window in main():
+# ceil(r_bits / w) fixed-base products plus 2 (into and out of Montgomery form).
 ALGO_MACS = {
-    "encrypt_i64": 28 * M(128),             # ceil(256/10) + 2 products, r < 2^256
+    "encrypt_i64": 28 * M(128),             # w = 10 (the reference's window), r < 2^256
     "mulmod": M(128),                       # ct + ct
     "pow_i64": 44 * M(128),                 # ct x int32 (algorithmic count: 44 products)
     "dec_half": 2 * (1023 + 171 + 16) * M(64),  # both CRT exponentiations per element
@@ -83,14 +85,74 @@ def parse():
     return ap.parse_args()
 
 
+def host_info() -> dict:
+    """The host's CPU as this process sees it: logical CPUs, the CPUs it may
+    run on, the cgroup CPU quota and the CPU model."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        info["cgroup_cpu_quota"] = None if quota == "max" else int(quota) / int(period)
+    except (OSError, ValueError):
+        info["cgroup_cpu_quota"] = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return info
+
+
+def usable_cores() -> int:
+    """Every CPU this process may run on, bounded by the cgroup quota."""
+    info = host_info()
+    n = info["affinity_cpus"]
+    if info["cgroup_cpu_quota"]:
+        n = min(n, max(1, int(info["cgroup_cpu_quota"])))
+    return max(1, n)
+
+
+def cpu_rates_3072(threads: int) -> dict:
+    """The oracle's per-op rates for a seeded 3072-bit key on a bounded sample
+    (enc, dec, ct+ct, ct x int32), for the 3072-bit shares of configs 3-5."""
+    from heu_amd import engine as E
+    from oracle import oracle as O
+
+    kd = E.keygen(3072, 7)
+    key = O.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
+    key.build_table(10, 256)
+    rng = np.random.default_rng(3)
+    sizes = {"enc": 16384, "dec": 4096, "add": 65536, "ctpt": 8192}
+    n = max(sizes.values())
+    m = rng.integers(-(2**63), 2**63 - 1, size=n, dtype=np.int64, endpoint=True)
+    r = E.seeded_r(7, 11, n, 256)
+    k = rng.integers(-(2**31), 2**31 - 1, size=n, dtype=np.int64, endpoint=True)
+    ct = key.encrypt_i64(m[: sizes["enc"]], r[: sizes["enc"]], use_table=True, threads=threads)
+    ct = np.concatenate([ct] * (n // len(ct)))
+    rev = ct[::-1].copy()
+    ops = {
+        "enc": lambda x: key.encrypt_i64(m[:x], r[:x], use_table=True, threads=threads),
+        "dec": lambda x: key.decrypt(ct[:x], out_words=2, threads=threads),
+        "add": lambda x: key.add(ct[:x], rev[:x], threads=threads),
+        "ctpt": lambda x: key.mul_i64(ct[:x], k[:x], threads=threads),
+    }
+    rates = {}
+    for name, fn in ops.items():
+        t = time.perf_counter()
+        fn(sizes[name])
+        rates[name] = sizes[name] / (time.perf_counter() - t)
+    return {"rates": rates, "sample": sizes}
+
+
 def cpu_baseline(kd, m, r, k):
     """Times the oracle on this host: every op on the whole sample (65,536
-    elements of the same workload) over all cores (capped at 16), plus each
-    op's single-core rate on a slice of it."""
+    elements of the same workload) over every usable core, plus each op's
+    single-core rate on a slice of it."""
     from oracle import oracle as O
 
     O.build()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = usable_cores()
     key = O.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
     key.build_table(10, 256)
     n = len(m)
@@ -117,17 +179,35 @@ def cpu_baseline(kd, m, r, k):
         single[name] = x / (time.perf_counter() - t)
         single_n[name] = x
     chain = 1.0 / sum(1.0 / rates[x] for x in ("enc", "add", "ctpt", "dec"))
+    r3072 = cpu_rates_3072(threads)
     return ct, {
         "value": chain,
         "unit": "elements/s (enc+add+ct*pt+dec)",
         "cores": threads,
         "kind": "port",
+        "host": host_info(),
         "sample": f"{n} elements of the same workload per op (enc, add, ct*pt, dec), {threads} threads "
-        "(std::thread-style chunking per op); single-core rates on "
-        + ", ".join(f"{x} {single_n[x]}" for x in single_n) + " elements",
+        "(one contiguous chunk per thread per op); single-core rates on "
+        + ", ".join(f"{x} {single_n[x]}" for x in single_n) + " elements; 3072-bit rates on "
+        + ", ".join(f"{x} {v}" for x, v in r3072["sample"].items()) + " elements",
         "per_op": rates,
         "single_core_per_op": single,
+        "per_op_3072": r3072["rates"],
     }
+
+
+def cpu_work_seconds(work: dict, cpu: dict) -> float:
+    """CPU seconds for a workload given as {op: {key_bits: elements}}, at the
+    oracle's all-core rates: a dot term is a ct x int32 and a ct+ct (numpy
+    MatMul's Mul then AddInplace), a sum term one ct+ct."""
+    secs = 0.0
+    for op, by_bits in work.items():
+        for bits, count in by_bits.items():
+            rt = cpu["per_op"] if int(bits) == 2048 else cpu["per_op_3072"]
+            per = {"enc": 1 / rt["enc"], "dec": 1 / rt["dec"], "add": 1 / rt["add"], "mul": 1 / rt["ctpt"],
+                   "dot": 1 / rt["ctpt"] + 1 / rt["add"], "sum": 1 / rt["add"]}[op]
+            secs += count * per
+    return secs
 
 
 def main():
@@ -153,6 +233,8 @@ def main():
     key = E.Key.from_keygen(ctx, kd, r_bits=256, window_bits=0)  # widest table under 1 GiB
     # table window, decryption lanes, work-list grid: the fastest on this device
     tune = None if args.no_autotune else key.autotune()
+    # encryption's algorithmic work at the window the key runs with
+    ALGO_MACS["encrypt_i64"] = (-(-256 // key.table_window) + 2) * M(128)
     rng = np.random.default_rng(0)
     m_h = rng.integers(-(2**63), 2**63 - 1, size=N, dtype=np.int64, endpoint=True)
     r_h = E.seeded_r(0, 11, N, 256)
@@ -250,6 +332,9 @@ def main():
             if name in traffic_all:  # HBM bytes per 131,072-element launch, from the counter profile
                 row["hbm_gb_per_s"] = traffic_all[name] * (elems / launches / 131072) / (total / launches / 1e3) / 1e9
         kernels[name] = row
+    over = [name for name, row in kernels.items() if row["valu_frac"] > 1.0]
+    if over:
+        print(f"[bench] algorithmic rate above the VALU peak for {over}: the work count is wrong", file=sys.stderr)
     dom = max(kern, key=lambda n: kern[n][0])
     dom_ms, dom_launches = kern[dom]
     elems_per_launch = (N * args.steps) / dom_launches
@@ -320,6 +405,12 @@ def main():
     }
     if cpu:
         line["cpu_baseline"]["speedup_gpu_over_cpu"] = value / cpu["value"]
+        sp = {op: per_op_rate[op] / cpu["per_op"][op] for op in ops}
+        line["cpu_baseline"]["speedup_per_op"] = sp
+        # the north star's bar: >= 50x the all-core host rate for encrypt and decrypt
+        line["cpu_baseline"]["target_50x"] = {op: sp[op] >= 50.0 for op in ("enc", "dec")}
+        line["cpu_baseline"]["speedup_per_core_per_op"] = {
+            op: per_op_rate[op] / cpu["single_core_per_op"][op] for op in ops}
         line["parity_subset"] = parity
     extra = {}
     wanted = [c.strip() for c in args.configs.split(",") if c.strip()]
@@ -332,6 +423,16 @@ def main():
             t = time.perf_counter()
             print(f"[bench] config {c} ...", file=sys.stderr, flush=True)
             extra[c] = getattr(C, f"run_config{c}")(ctx)
+            if cpu and "work" in extra[c]:
+                # the oracle's all-core time for the same workload, from its per-op rates
+                secs = cpu_work_seconds(extra[c]["work"], cpu)
+                extra[c]["cpu_all_core_s"] = secs
+                extra[c]["cpu_cores"] = cpu["cores"]
+                if "batch_graph_ms" in extra[c]:
+                    extra[c]["speedup_batch_vs_cpu"] = secs / (extra[c]["batch_graph_ms"] / 1e3)
+                    extra[c]["speedup_plain_vs_cpu"] = secs / (extra[c]["plain_ms"] / 1e3)
+                if "requests_in_sweep" in extra[c]:
+                    extra[c]["cpu_capacity_req_per_s"] = extra[c]["requests_in_sweep"] / secs
             print(f"[bench] config {c} done in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
             torch.cuda.empty_cache()
     if "5" in extra:
@@ -344,6 +445,15 @@ def main():
         line["served_at_req_per_s"] = pick["offered_req_per_s"]
         line["plain_fifo_p50_ms"] = plain["p50_ms"]
         line["plain_fifo_p99_ms"] = plain["p99_ms"]
+        pb = next(x for x in extra["5"]["plain_batched"] if x["offered_req_per_s"] == pick["offered_req_per_s"])
+        line["plain_batched_p99_ms"] = pb["p99_ms"]
+        line["plain_batched_batch_ms_mean"] = pb["batch_ms_mean"]
+        # the north star's serving bar: served p99 <= 2x the plain batched path's per-batch time
+        # per-batch time as the mean and as the p99 of the plain batches (their
+        # sizes span five orders of magnitude, so the two differ by 10-20x)
+        for stat in ("mean", "p99"):
+            met = [x["offered_req_per_s"] for x in kept if x[f"p99_over_2x_plain_batch_{stat}"] <= 1.0]
+            line[f"served_highest_rate_p99_within_2x_batch_{stat}"] = max(met) if met else None
     if extra:
         line["configs"] = extra
     print(json.dumps(line))
