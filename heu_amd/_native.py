@@ -61,6 +61,15 @@ SIGNATURES = {
     "heu_server_wait": (I, [P, U64, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
     "heu_server_stats": (I, [P, P]),
     "heu_server_destroy": (I, [P]),
+    "heu_wire_put_ciphertext": (I, [P, U32, P, SZ, ctypes.POINTER(SZ)]),
+    "heu_wire_get_ciphertext": (I, [P, SZ, P, U32]),
+    "heu_wire_put_matrix": (I, [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, U32, P, SZ, ctypes.POINTER(SZ)]),
+    "heu_wire_get_matrix": (I, [P, SZ, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                ctypes.POINTER(ctypes.c_int64), P, U32, SZ]),
+    "heu_wire_put_public_key": (I, [P, U32, P, P, SZ, ctypes.POINTER(SZ)]),
+    "heu_wire_get_public_key": (I, [P, SZ, P, P, U32, ctypes.POINTER(U32)]),
+    "heu_wire_put_secret_key": (I, [P, P, U32, P, SZ, ctypes.POINTER(SZ)]),
+    "heu_wire_get_secret_key": (I, [P, SZ, P, P, U32, ctypes.POINTER(U32)]),
 }
 
 # heu_batch ops and the heu_task struct (include/heu_paillier.h)
@@ -69,6 +78,7 @@ SIGNATURES = {
 HEU_BATCH_GRAPH = 1
 HEU_PARAM_TABLE_WINDOW, HEU_PARAM_DEC_LANES = 1, 2
 HEU_TUNE_TABLE, HEU_TUNE_DEC_LANES, HEU_TUNE_GRID, HEU_TUNE_ALL = 1, 2, 4, 7
+HEU_SCHEMA_ZPAILLIER = 4
 
 
 class HeuTask(ctypes.Structure):

@@ -4,8 +4,9 @@ Sources: csrc/ops_{2048,3072}.hip (kernels and launch tables per key size) and
 csrc/worklist_{,dec_}{2048,3072}.hip (persistent work-list kernels), all built from
 csrc/paillier_device.hpp; csrc/engine.cpp (contexts, keys, one-op entry
 points), csrc/batch.cpp (multi-task batches, graphs), csrc/server.cpp (request
-server), csrc/tune.cpp (autotuner), csrc/keygen.cpp and csrc/noalloc.hip
-(heu_sum_noalloc and its fixed device scratch).
+server), csrc/tune.cpp (autotuner), csrc/keygen.cpp, csrc/noalloc.hip
+(heu_sum_noalloc and its fixed device scratch) and csrc/wire.cpp (HEU's
+serialized form of ciphertexts, matrices and keys).
 The library links OpenSSL's libcrypto for per-key host constants.
 """
 from __future__ import annotations
@@ -22,7 +23,7 @@ ARCH = os.environ.get("HEU_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["ops_2048.hip", "ops_3072.hip", "worklist_2048.hip", "worklist_3072.hip", "worklist_dec_2048.hip",
            "worklist_dec_3072.hip", "engine.cpp", "batch.cpp", "server.cpp", "tune.cpp",
-           "keygen.cpp", "noalloc.hip"]
+           "keygen.cpp", "noalloc.hip", "wire.cpp"]
 # Packed FP32 instructions stay off in device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -1,0 +1,388 @@
+// wire.cpp -- HEU's serialized form of ZPaillier objects (heu_wire_* in
+// include/heu_paillier.h).  Host code only: ciphertexts arrive here in the
+// Montgomery form heu_to_mont writes on the device.
+//
+// The layout follows the reference's writers:
+//   - paillier_z::Ciphertext is MSGPACK_DEFINE(c_) (paillier_zahlen/
+//     ciphertext.h:37): a msgpack array holding the BigInt c_;
+//   - PublicKey packs [n, h_s] (public_key.h:64-76), SecretKey
+//     [lambda, mu, p, q] (secret_key.h:64-80);
+//   - phe's SerializableVariant appends the schema index as a little-endian
+//     size_t (phe/base/serializable_types.cc:50-70), 4 for ZPaillier
+//     (phe/base/schema.h:49);
+//   - DenseMatrix packs [rows, cols, ndim, [items]] with every item's bytes as
+//     a msgpack str, items in Eigen's column-major order (numpy/matrix.h:221-265).
+// A BigInt is packed by yacl's msgpack adaptor, which the reference does not
+// vendor; here it is a msgpack str holding one sign byte (0 non-negative,
+// 1 negative) and the magnitude's bytes, little-endian, without high zero
+// bytes.  Integers use msgpack's shortest encoding, as msgpack-c writes them.
+#include <openssl/bn.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/heu_paillier.h"
+#include "engine_internal.hpp"
+
+using heu_engine::fail;
+
+namespace {
+
+// Writes into [p, p + cap); with p == nullptr it only counts.  `n` is the
+// length so far whether or not it fit.
+struct Out {
+  uint8_t* p;
+  size_t cap, n = 0;
+  void byte(uint8_t b) {
+    if (p && n < cap) p[n] = b;
+    ++n;
+  }
+  void bytes(const uint8_t* b, size_t len) {
+    if (p && n + len <= cap) std::memcpy(p + n, b, len);
+    n += len;
+  }
+  void be(uint64_t v, int width) {
+    for (int i = width - 1; i >= 0; --i) byte((uint8_t)(v >> (8 * i)));
+  }
+  void array(uint32_t count) {
+    if (count < 16) byte((uint8_t)(0x90 | count));
+    else if (count < 65536) { byte(0xdc); be(count, 2); }
+    else { byte(0xdd); be(count, 4); }
+  }
+  void str_header(size_t len) {
+    if (len < 32) byte((uint8_t)(0xa0 | len));
+    else if (len < 256) { byte(0xd9); be(len, 1); }
+    else if (len < 65536) { byte(0xda); be(len, 2); }
+    else { byte(0xdb); be(len, 4); }
+  }
+  void integer(int64_t v) {
+    if (v >= 0) {
+      const uint64_t u = (uint64_t)v;
+      if (u < 128) byte((uint8_t)u);
+      else if (u < 256) { byte(0xcc); be(u, 1); }
+      else if (u < 65536) { byte(0xcd); be(u, 2); }
+      else if (u < (1ull << 32)) { byte(0xce); be(u, 4); }
+      else { byte(0xcf); be(u, 8); }
+    } else if (v >= -32) {
+      byte((uint8_t)(0xe0 | (v & 0x1f)));
+    } else if (v >= -128) {
+      byte(0xd0); be((uint64_t)v, 1);
+    } else if (v >= -32768) {
+      byte(0xd1); be((uint64_t)v, 2);
+    } else if (v >= INT32_MIN) {
+      byte(0xd2); be((uint64_t)v, 4);
+    } else {
+      byte(0xd3); be((uint64_t)v, 8);
+    }
+  }
+  // A non-negative BigInt given as `words` little-endian uint32 limbs.
+  void bigint_words(const uint32_t* w, size_t words) {
+    size_t nb = words * 4;
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(w);  // little-endian host
+    while (nb > 0 && b[nb - 1] == 0) --nb;
+    str_header(nb + 1);
+    byte(0);
+    bytes(b, nb);
+  }
+  void bigint(const BIGNUM* v) {
+    std::vector<uint8_t> mag((size_t)BN_num_bytes(v));
+    if (!mag.empty()) BN_bn2lebinpad(v, mag.data(), (int)mag.size());
+    str_header(mag.size() + 1);
+    byte(BN_is_negative(v) ? 1 : 0);
+    bytes(mag.data(), mag.size());
+  }
+  void schema_index() {
+    const uint64_t idx = HEU_SCHEMA_ZPAILLIER;
+    for (int i = 0; i < 8; ++i) byte((uint8_t)(idx >> (8 * i)));
+  }
+};
+
+int finish(const Out& o, size_t* len) {
+  if (len) *len = o.n;
+  if (!o.p || o.n > o.cap) return fail(HEU_ERANGE, "output buffer holds %zu bytes; %zu are needed", o.cap, o.n);
+  return HEU_OK;
+}
+
+// Reads the msgpack subset written above (and bin / every integer form).
+struct In {
+  const uint8_t* p;
+  size_t len, at = 0;
+  bool ok = true;
+  uint8_t byte() {
+    if (at >= len) {
+      ok = false;
+      return 0;
+    }
+    return p[at++];
+  }
+  uint64_t be(int width) {
+    uint64_t v = 0;
+    for (int i = 0; i < width; ++i) v = (v << 8) | byte();
+    return v;
+  }
+  bool array(uint32_t* count) {
+    const uint8_t t = byte();
+    if ((t & 0xf0) == 0x90) *count = t & 0x0f;
+    else if (t == 0xdc) *count = (uint32_t)be(2);
+    else if (t == 0xdd) *count = (uint32_t)be(4);
+    else return ok = false;
+    return ok;
+  }
+  // str or bin: its body in [*body, *body + *n)
+  bool blob(const uint8_t** body, size_t* n) {
+    const uint8_t t = byte();
+    if ((t & 0xe0) == 0xa0) *n = t & 0x1f;
+    else if (t == 0xd9 || t == 0xc4) *n = (size_t)be(1);
+    else if (t == 0xda || t == 0xc5) *n = (size_t)be(2);
+    else if (t == 0xdb || t == 0xc6) *n = (size_t)be(4);
+    else return ok = false;
+    if (!ok || *n > len - at) return ok = false;
+    *body = p + at;
+    at += *n;
+    return true;
+  }
+  bool integer(int64_t* v) {
+    const uint8_t t = byte();
+    if (t < 0x80) *v = t;
+    else if (t >= 0xe0) *v = (int8_t)t;
+    else if (t == 0xcc) *v = (int64_t)be(1);
+    else if (t == 0xcd) *v = (int64_t)be(2);
+    else if (t == 0xce) *v = (int64_t)be(4);
+    else if (t == 0xcf) {
+      const uint64_t u = be(8);
+      if (u >> 63) return ok = false;
+      *v = (int64_t)u;
+    } else if (t == 0xd0) *v = (int8_t)be(1);
+    else if (t == 0xd1) *v = (int16_t)be(2);
+    else if (t == 0xd2) *v = (int32_t)be(4);
+    else if (t == 0xd3) *v = (int64_t)be(8);
+    else return ok = false;
+    return ok;
+  }
+  // A BigInt into BIGNUM r.
+  bool bigint(BIGNUM* r) {
+    const uint8_t* b;
+    size_t n;
+    if (!blob(&b, &n) || n < 1 || b[0] > 1) return ok = false;
+    BN_lebin2bn(b + 1, (int)(n - 1), r);
+    BN_set_negative(r, b[0] == 1 && !BN_is_zero(r));
+    return true;
+  }
+  // A non-negative BigInt into `words` limbs (fails if it does not fit).
+  bool bigint_words(uint32_t* w, size_t words) {
+    const uint8_t* b;
+    size_t n;
+    if (!blob(&b, &n) || n < 1 || b[0] != 0) return ok = false;
+    // b[1..n) is the magnitude: bytes beyond `words` limbs must be zero
+    if (n - 1 > words * 4) {
+      for (size_t i = words * 4; i < n - 1; ++i)
+        if (b[1 + i]) return ok = false;
+    }
+    std::memset(w, 0, words * 4);
+    std::memcpy(w, b + 1, std::min(n - 1, words * 4));
+    return true;
+  }
+  bool schema_index_at_end(size_t* payload_len) {
+    if (len < 9) return false;
+    uint64_t idx = 0;
+    for (int i = 7; i >= 0; --i) idx = (idx << 8) | p[len - 8 + i];
+    if (idx != HEU_SCHEMA_ZPAILLIER) return false;
+    *payload_len = len - 8;
+    return true;
+  }
+};
+
+struct Bn {
+  BIGNUM* b = BN_new();
+  Bn() = default;
+  Bn(const Bn&) = delete;
+  ~Bn() { BN_free(b); }
+  operator BIGNUM*() const { return b; }
+};
+
+void put_ciphertext(Out& o, const uint32_t* mont, uint32_t words) {
+  o.array(1);  // MSGPACK_DEFINE(c_)
+  o.bigint_words(mont, words);
+  o.schema_index();
+}
+
+size_t ciphertext_len(const uint32_t* mont, uint32_t words) {
+  Out o{nullptr, 0};
+  put_ciphertext(o, mont, words);
+  return o.n;
+}
+
+bool get_ciphertext(const uint8_t* b, size_t n, uint32_t* mont, uint32_t words) {
+  In in{b, n};
+  size_t plen;
+  if (!in.schema_index_at_end(&plen)) return false;
+  in.len = plen;
+  uint32_t cnt = 0;
+  return in.array(&cnt) && cnt == 1 && in.bigint_words(mont, words) && in.at == in.len;
+}
+
+int to_words(const BIGNUM* v, uint32_t* w, uint32_t words) {
+  if (BN_is_negative(v) || BN_num_bytes(v) > (int)(4 * words))
+    return fail(HEU_ERANGE, "a key value does not fit %u words", words);
+  BN_bn2lebinpad(v, reinterpret_cast<unsigned char*>(w), (int)(4 * words));
+  return HEU_OK;
+}
+
+uint32_t words_of(const BIGNUM* v) { return (uint32_t)((BN_num_bits(v) + 31) / 32); }
+
+}  // namespace
+
+extern "C" {
+
+int heu_wire_put_ciphertext(const uint32_t* mont, uint32_t words, uint8_t* out, size_t cap, size_t* len) {
+  if (!mont || words == 0 || !len) return fail(HEU_EINVAL, "null argument");
+  Out o{out, cap};
+  put_ciphertext(o, mont, words);
+  return finish(o, len);
+}
+
+int heu_wire_get_ciphertext(const uint8_t* in, size_t len, uint32_t* mont, uint32_t words) {
+  if (!in || !mont || words == 0) return fail(HEU_EINVAL, "null argument");
+  if (!get_ciphertext(in, len, mont, words))
+    return fail(HEU_EINVAL, "not a serialized ZPaillier ciphertext of at most %u words", words);
+  return HEU_OK;
+}
+
+int heu_wire_put_matrix(const uint32_t* mont, int64_t rows, int64_t cols, int64_t ndim, uint32_t words,
+                        uint8_t* out, size_t cap, size_t* len) {
+  if (!len || words == 0 || rows < 0 || cols < 0 || ndim < 0 || ndim > 2) return fail(HEU_EINVAL, "bad argument");
+  const int64_t count = rows * cols;
+  if (count > 0 && !mont) return fail(HEU_EINVAL, "null argument");
+  if (count > (int64_t)UINT32_MAX) return fail(HEU_EINVAL, "too many elements for a msgpack array");
+  Out o{out, cap};
+  o.array(4);
+  o.integer(rows);
+  o.integer(cols);
+  o.integer(ndim);
+  o.array((uint32_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    const uint32_t* v = mont + (size_t)i * words;
+    o.str_header(ciphertext_len(v, words));
+    put_ciphertext(o, v, words);
+  }
+  return finish(o, len);
+}
+
+int heu_wire_get_matrix(const uint8_t* in, size_t len, int64_t* rows, int64_t* cols, int64_t* ndim, uint32_t* mont,
+                        uint32_t words, size_t cap_values) {
+  if (!in) return fail(HEU_EINVAL, "null argument");
+  In r{in, len};
+  uint32_t four = 0, count = 0;
+  int64_t nr = 0, nc = 0, nd = 0;
+  if (!r.array(&four) || four != 4 || !r.integer(&nr) || !r.integer(&nc) || !r.integer(&nd) || !r.array(&count) ||
+      nr < 0 || nc < 0 || nd < 0 || nd > 2 || (uint64_t)nr * (uint64_t)nc != count)
+    return fail(HEU_EINVAL, "not a serialized ciphertext matrix: [rows, cols, ndim, items] expected");
+  if (rows) *rows = nr;
+  if (cols) *cols = nc;
+  if (ndim) *ndim = nd;
+  if (!mont) return HEU_OK;
+  if (words == 0) return fail(HEU_EINVAL, "words must be positive");
+  if (cap_values < count) return fail(HEU_ERANGE, "the matrix holds %u values; room for %zu", count, cap_values);
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint8_t* b;
+    size_t n;
+    if (!r.blob(&b, &n) || !get_ciphertext(b, n, mont + (size_t)i * words, words))
+      return fail(HEU_EINVAL, "matrix item %u is not a serialized ZPaillier ciphertext of at most %u words", i, words);
+  }
+  if (r.at != len) return fail(HEU_EINVAL, "trailing bytes after the matrix");
+  return HEU_OK;
+}
+
+int heu_wire_put_public_key(const uint32_t* n, uint32_t n_words, const uint32_t* h_s, uint8_t* out, size_t cap,
+                            size_t* len) {
+  if (!n || !h_s || n_words == 0 || !len) return fail(HEU_EINVAL, "null argument");
+  Out o{out, cap};
+  o.array(2);
+  o.bigint_words(n, n_words);
+  o.bigint_words(h_s, 2 * (size_t)n_words);
+  o.schema_index();
+  return finish(o, len);
+}
+
+int heu_wire_get_public_key(const uint8_t* in, size_t len, uint32_t* n, uint32_t* h_s, uint32_t cap_words,
+                            uint32_t* n_words) {
+  if (!in || !n_words) return fail(HEU_EINVAL, "null argument");
+  In r{in, len};
+  size_t plen;
+  if (!r.schema_index_at_end(&plen)) return fail(HEU_EINVAL, "not a serialized ZPaillier public key");
+  r.len = plen;
+  Bn bn_n, bn_h;
+  uint32_t two = 0;
+  if (!r.array(&two) || two != 2 || !r.bigint(bn_n) || !r.bigint(bn_h) || r.at != r.len || BN_is_negative(bn_n) ||
+      BN_is_negative(bn_h))
+    return fail(HEU_EINVAL, "not a serialized ZPaillier public key: [n, h_s] expected");
+  *n_words = words_of(bn_n);
+  if (!n || !h_s) return HEU_OK;
+  if (cap_words < *n_words) return fail(HEU_ERANGE, "n has %u words; room for %u", *n_words, cap_words);
+  int rc = to_words(bn_n, n, cap_words);
+  if (!rc) rc = to_words(bn_h, h_s, 2 * cap_words);
+  return rc;
+}
+
+int heu_wire_put_secret_key(const uint32_t* p, const uint32_t* q, uint32_t prime_words, uint8_t* out, size_t cap,
+                            size_t* len) {
+  if (!p || !q || prime_words == 0 || !len) return fail(HEU_EINVAL, "null argument");
+  BN_CTX* ctx = BN_CTX_new();
+  Bn bp, bq, n, lam, mu, t;
+  BN_lebin2bn(reinterpret_cast<const unsigned char*>(p), (int)(4 * prime_words), bp);
+  BN_lebin2bn(reinterpret_cast<const unsigned char*>(q), (int)(4 * prime_words), bq);
+  BN_mul(n, bp, bq, ctx);
+  // key_generator.cc:26-60: lambda = (p - 1)(q - 1) / 2, mu = lambda^-1 mod n
+  BN_sub(lam, bp, BN_value_one());
+  BN_sub(t, bq, BN_value_one());
+  BN_mul(lam, lam, t, ctx);
+  BN_rshift1(lam, lam);
+  const bool inv = BN_mod_inverse(mu, lam, n, ctx) != nullptr;
+  BN_CTX_free(ctx);
+  if (!inv) return fail(HEU_EKEY, "lambda has no inverse mod n: p, q are not a Paillier key");
+  Out o{out, cap};
+  o.array(4);
+  o.bigint(lam);
+  o.bigint(mu);
+  o.bigint(bp);
+  o.bigint(bq);
+  o.schema_index();
+  return finish(o, len);
+}
+
+int heu_wire_get_secret_key(const uint8_t* in, size_t len, uint32_t* p, uint32_t* q, uint32_t cap_words,
+                            uint32_t* prime_words) {
+  if (!in || !prime_words) return fail(HEU_EINVAL, "null argument");
+  In r{in, len};
+  size_t plen;
+  if (!r.schema_index_at_end(&plen)) return fail(HEU_EINVAL, "not a serialized ZPaillier secret key");
+  r.len = plen;
+  Bn lam, mu, bp, bq;
+  uint32_t four = 0;
+  if (!r.array(&four) || four != 4 || !r.bigint(lam) || !r.bigint(mu) || !r.bigint(bp) || !r.bigint(bq) ||
+      r.at != r.len)
+    return fail(HEU_EINVAL, "not a serialized ZPaillier secret key: [lambda, mu, p, q] expected");
+  // the packed lambda and mu must be those of p and q
+  BN_CTX* ctx = BN_CTX_new();
+  Bn n, l2, t, m2;
+  BN_mul(n, bp, bq, ctx);
+  BN_sub(l2, bp, BN_value_one());
+  BN_sub(t, bq, BN_value_one());
+  BN_mul(l2, l2, t, ctx);
+  BN_rshift1(l2, l2);
+  const bool inv = BN_mod_inverse(m2, l2, n, ctx) != nullptr;
+  BN_CTX_free(ctx);
+  if (!inv || BN_cmp(lam, l2) != 0 || BN_cmp(mu, m2) != 0)
+    return fail(HEU_EKEY, "lambda and mu do not belong to p and q");
+  *prime_words = std::max(words_of(bp), words_of(bq));
+  if (!p || !q) return HEU_OK;
+  if (cap_words < *prime_words) return fail(HEU_ERANGE, "the primes have %u words; room for %u", *prime_words, cap_words);
+  int rc = to_words(bp, p, cap_words);
+  if (!rc) rc = to_words(bq, q, cap_words);
+  return rc;
+}
+
+}  // extern "C"

@@ -22,6 +22,20 @@ import numpy as np
 import torch
 
 from . import engine as E
+from . import wire
+
+
+class MatrixSerializeFormat:
+    """numpy/matrix.h MatrixSerializeFormat: Best is the msgpack form;
+    Interconnection is the org_interconnection protobuf, whose schema is not in
+    the reference tree and is not built here."""
+
+    Best = "best"
+    Interconnection = "interconnection"
+
+
+# yacl's BigInt backends (OpenSSL, GMP) use 64-bit words: R = 2^(64 ceil(bits / 64))
+MONT_WORD_BITS = 64
 
 
 class CiphertextArray:
@@ -84,6 +98,48 @@ class CiphertextArray:
     def __repr__(self) -> str:
         return f"CiphertextArray(shape={self.shape}, key_bits={self.key.key_bits})"
 
+    # ------------------------------------------------------------ wire format
+    def _colmajor(self) -> np.ndarray:
+        """Positions of `data` in Eigen's column-major order of (rows, cols)."""
+        return self._colmajor_of(self.rows(), self.cols())
+
+    def serialize(self, format: str = MatrixSerializeFormat.Best) -> bytes:
+        """DenseMatrix<Ciphertext>::Serialize (numpy/matrix.h:221-265): every
+        ciphertext in the Montgomery form the reference stores, column-major."""
+        if format != MatrixSerializeFormat.Best:
+            raise NotImplementedError("the interconnection protobuf schema (org_interconnection) is not built")
+        key = self.key
+        if self.size == 0:
+            mont = np.zeros((0, key.ct_words), np.uint32)
+        else:
+            mont = key.to_mont(self.data, MONT_WORD_BITS).cpu().numpy().view(np.uint32)
+            mont = mont[self._colmajor()]
+        return wire.put_matrix(mont, self.rows(), self.cols(), self.ndim)
+
+    @staticmethod
+    def load_from(key, buf: bytes, format: str = MatrixSerializeFormat.Best) -> "CiphertextArray":
+        """DenseMatrix<Ciphertext>::LoadFrom (numpy/matrix.h:267-314) onto the
+        device of `key` (an engine Key or a HeKit)."""
+        if format != MatrixSerializeFormat.Best:
+            raise NotImplementedError("the interconnection protobuf schema (org_interconnection) is not built")
+        key = key.key if isinstance(key, HeKit) else key
+        rows, cols, ndim, mont = wire.get_matrix(bytes(buf), key.ct_words)
+        shape = () if ndim == 0 else ((rows,) if ndim == 1 else (rows, cols))
+        if int(np.prod(shape, dtype=np.int64)) != rows * cols:
+            raise ValueError(f"a {ndim}-d matrix cannot hold {rows} x {cols} elements")
+        if rows * cols == 0:
+            return CiphertextArray(key, key._ct_out(0), shape)
+        z = key.from_mont(mont, MONT_WORD_BITS)
+        # column-major item k holds C-order element (k % rows) * cols + k // rows
+        inv = np.empty(rows * cols, dtype=np.int64)
+        inv[CiphertextArray._colmajor_of(rows, cols)] = np.arange(rows * cols, dtype=np.int64)
+        return CiphertextArray(key, z.index_select(0, torch.from_numpy(inv).to(z.device)), shape)
+
+    @staticmethod
+    def _colmajor_of(rows: int, cols: int) -> np.ndarray:
+        k = np.arange(rows * cols, dtype=np.int64)
+        return (k % rows) * cols + k // rows
+
 
 def _is_ct(x) -> bool:
     return isinstance(x, CiphertextArray)
@@ -120,6 +176,30 @@ class Encryptor:
         ct = self.kit.key.encrypt_i64(flat, self._r(flat.size, seed))
         return CiphertextArray(self.kit.key, ct, a.shape)
 
+    def encrypt_with_audit(self, pts, seed=None):
+        """(ciphertexts, audit strings) as numpy/encryptor.cc:107-121 and
+        paillier_zahlen/encryptor.cc:50-54 build them: "p:{m},rn:{h_s^r},c:{c}"
+        per element, hex, with rn and c in the Montgomery form the reference
+        stores (R for 64-bit words)."""
+        a = _pt(pts)
+        flat = np.ascontiguousarray(a.ravel())
+        key = self.kit.key
+        if flat.size == 0:
+            return CiphertextArray(key, key._ct_out(0), a.shape), np.zeros(a.shape, dtype=object)
+        r = self._r(flat.size, seed)
+        ct = key.encrypt_i64(flat, r)
+        rn = key.randomize(None, r)  # EncryptZero with the same r: h_s^r
+        c_m = key.to_mont(ct, MONT_WORD_BITS).cpu().numpy().view(np.uint32)
+        rn_m = key.to_mont(rn, MONT_WORD_BITS).cpu().numpy().view(np.uint32)
+        audit = np.array([f"p:{_hex(int(m))},rn:{_hex(E.words_to_int(x))},c:{_hex(E.words_to_int(y))}"
+                          for m, x, y in zip(flat, rn_m, c_m)], dtype=object).reshape(a.shape)
+        return CiphertextArray(key, ct, a.shape), audit
+
+
+def _hex(v: int) -> str:
+    """A BigInt's hex string: upper case, no leading zeros, '-' for negatives."""
+    return ("-" if v < 0 else "") + format(abs(v), "X")
+
 
 class Decryptor:
     """Decrypts to numpy arrays: int64 when every value fits, else Python ints
@@ -139,6 +219,33 @@ class Decryptor:
         wide = key.decrypt(ct.data, out_words=key.n_words).cpu().numpy().view(np.uint32)
         objs = np.array([E.twos_to_int(row) for row in wide], dtype=object)
         return objs.reshape(ct.shape)
+
+    def decrypt_in_range(self, ct: CiphertextArray, range_bits: int = 128):
+        """Decrypt, and throw unless every plaintext lies in (-2^range_bits,
+        2^range_bits), i.e. has at most range_bits bits (numpy/decryptor.h:37,
+        numpy/decryptor.cc:19-45; the check blocks plaintext-overflow attacks)."""
+        key = self.kit.key
+        if ct.size == 0:
+            return np.zeros(ct.shape, dtype=np.int64)
+        # room for range_bits of magnitude plus the sign, at most the plaintext width
+        ow = min(key.n_words, range_bits // 32 + 2)
+        out, ovf = key.decrypt(ct.data, out_words=ow, overflow=True)
+        rows = out.cpu().numpy().view(np.uint32)
+        bad = np.nonzero(ovf.cpu().numpy())[0]
+        vals = [E.twos_to_int(r) for r in rows] if ow > 2 else list(rows.view(np.int64).ravel())
+        if bad.size == 0:
+            over = [i for i, v in enumerate(vals) if abs(int(v)).bit_length() > range_bits]
+        else:
+            over = [int(bad[0])]
+        if over:
+            i = over[0]
+            found = abs(int(vals[i])).bit_length() if bad.size == 0 else f"> {32 * ow - 1}"
+            raise RuntimeError("Dangerous!!! HE ciphertext range check failed, there may be a malicious party "
+                               "stealing your data, please stop computing immediately. "
+                               f"found pt.BitCount()={found}, expected {range_bits}")
+        if all(-(2**63) <= int(v) < 2**63 for v in vals):
+            return np.array([int(v) for v in vals], dtype=np.int64).reshape(ct.shape)
+        return np.array([int(v) for v in vals], dtype=object).reshape(ct.shape)
 
 
 class Evaluator:
@@ -342,6 +449,51 @@ class Evaluator:
         return out
 
 
+class PublicKey:
+    """phe.PublicKey of the ZPaillier schema: n and h_s."""
+
+    def __init__(self, n: int, h_s: int):
+        self.n, self.h_s = int(n), int(h_s)
+
+    def serialize(self) -> bytes:
+        """msgpack [n, h_s] + schema index (paillier_zahlen/public_key.h:64-76)."""
+        return wire.put_public_key(self.n, self.h_s)
+
+    @staticmethod
+    def load_from(buf: bytes) -> "PublicKey":
+        return PublicKey(*wire.get_public_key(bytes(buf)))
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, PublicKey) and (self.n, self.h_s) == (other.n, other.h_s)
+
+
+class SecretKey:
+    """phe.SecretKey of the ZPaillier schema: p and q (lambda and mu derive
+    from them)."""
+
+    def __init__(self, p: int, q: int):
+        self.p, self.q = int(p), int(q)
+
+    @property
+    def lambda_(self) -> int:
+        return (self.p - 1) * (self.q - 1) // 2
+
+    @property
+    def mu(self) -> int:
+        return pow(self.lambda_, -1, self.p * self.q)
+
+    def serialize(self) -> bytes:
+        """msgpack [lambda, mu, p, q] + schema index (paillier_zahlen/secret_key.h:64-80)."""
+        return wire.put_secret_key(self.p, self.q)
+
+    @staticmethod
+    def load_from(buf: bytes) -> "SecretKey":
+        return SecretKey(*wire.get_secret_key(bytes(buf)))
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, SecretKey) and (self.p, self.q) == (other.p, other.q)
+
+
 class HeKit:
     """Keys resident on the device plus the three actors (numpy HeKit,
     bind_numpy.cc:242-293)."""
@@ -359,19 +511,32 @@ class HeKit:
     def evaluator(self) -> Evaluator:
         return self._ev
 
-    def public_key(self):
-        return self.key.n, self.key.hs
+    def public_key(self) -> PublicKey:
+        return PublicKey(self.key.n, self.key.hs)
+
+    def secret_key(self) -> SecretKey | None:
+        return SecretKey(self.key.p, self.key.q) if self.key.p is not None else None
 
     def array(self, ndarray) -> np.ndarray:
         """A plaintext array (numpy int64)."""
         return _pt(ndarray)
 
 
-def setup(key_bits: int = 2048, seed: int | None = None, ctx: E.Context | None = None,
-          r_bits: int | None = None, window_bits: int = 0) -> HeKit:
-    """Generates a key (seeded or from the OS RNG) and makes it resident.
+def setup(key_bits=2048, seed: int | None = None, ctx: E.Context | None = None,
+          r_bits: int | None = None, window_bits: int = 0, secret_key: SecretKey | None = None) -> HeKit:
+    """Generates a key (seeded or from the OS RNG) and makes it resident; or,
+    given a PublicKey (and optionally its SecretKey) as the first argument,
+    makes that key resident (hnp.setup(pk[, sk]) of bind_numpy.cc).
     r_bits defaults to key_bits / 2 like the reference's GetRn (encryptor.cc:27)."""
     ctx = ctx or E.Context(0)
+    if isinstance(key_bits, PublicKey):
+        pk = key_bits
+        bits = (pk.n.bit_length() + 31) // 32 * 32
+        p, q = (secret_key.p, secret_key.q) if secret_key is not None else (None, None)
+        if p is not None and p * q != pk.n:
+            raise ValueError("the secret key does not belong to the public key")
+        key = E.Key(ctx, pk.n, pk.h_s, p, q, r_bits=r_bits or bits // 2, window_bits=window_bits)
+        return HeKit(key)
     kd = E.keygen(key_bits, seed)
     key = E.Key.from_keygen(ctx, kd, r_bits=r_bits or key_bits // 2, window_bits=window_bits)
     return HeKit(key)

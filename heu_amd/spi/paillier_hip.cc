@@ -655,10 +655,118 @@ void write_back(Span<T> dst, std::vector<T>&& src) {
 
 std::string hex_words(const std::vector<uint32_t>& w) { return BigInt::FromWords(w.data(), w.size(), false).ToHexString(); }
 
+// x R or x R^-1 mod m for R = 2^(64 ceil(bits(m) / 64)): the Montgomery form of
+// yacl's 64-bit-word backends (MontgomerySpace::MapIntoMSpace / MapBackToZSpace).
+std::vector<uint32_t> mont_words(const std::vector<uint32_t>& x, const BigInt& m, bool into) {
+  BN_CTX* ctx = BN_CTX_new();
+  BIGNUM *R = BN_new(), *v = BN_new();
+  BN_set_bit(R, 64 * ((BN_num_bits(m.bn()) + 63) / 64));
+  BN_nnmod(R, R, m.bn(), ctx);
+  if (!into) BN_mod_inverse(R, R, m.bn(), ctx);
+  BN_lebin2bn(reinterpret_cast<const unsigned char*>(x.data()), (int)(4 * x.size()), v);
+  BN_mod_mul(v, v, R, m.bn(), ctx);
+  std::vector<uint32_t> out(x.size());
+  BN_bn2lebinpad(v, reinterpret_cast<unsigned char*>(out.data()), (int)(4 * out.size()));
+  BN_free(R);
+  BN_free(v);
+  BN_CTX_free(ctx);
+  return out;
+}
+
+// The 8-byte little-endian schema index HEU's variant wrapper appends
+// (phe/base/serializable_types.cc:50-70); the engine's wire readers expect it.
+constexpr uint8_t kSchemaIndex[8] = {HEU_SCHEMA_ZPAILLIER, 0, 0, 0, 0, 0, 0, 0};
+
+// Runs a heu_wire_put_* writer into a vector and drops the schema index: these
+// are the backend object's own bytes.
+template <class F>
+std::vector<uint8_t> wire_bytes(F&& put) {
+  size_t len = 0;
+  (void)put(nullptr, 0, &len);  // HEU_ERANGE, with the length
+  std::vector<uint8_t> out(len);
+  ck(put(out.data(), len, &len));
+  if (len < sizeof(kSchemaIndex)) throw_msg("wire writer returned too few bytes");
+  out.resize(len - sizeof(kSchemaIndex));
+  return out;
+}
+
+std::vector<uint8_t> with_index(std::span<const uint8_t> in) {
+  std::vector<uint8_t> v(in.begin(), in.end());
+  v.insert(v.end(), kSchemaIndex, kSchemaIndex + sizeof(kSchemaIndex));
+  return v;
+}
+
+// A non-negative value as `words` little-endian limbs (unsigned, unlike ToWords).
+std::vector<uint32_t> big_words(const BigInt& v, size_t words) {
+  if (v.IsNegative() || v.BitCount() > 32 * words) throw_msg("key value does not fit its limbs");
+  std::vector<uint32_t> w(words);
+  bn_check(BN_bn2lebinpad(v.bn(), reinterpret_cast<unsigned char*>(w.data()), (int)(4 * words)) >= 0, "bn2lebinpad");
+  return w;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ keys
 std::string Ciphertext::ToString() const { return hex_words(c_); }
+
+std::vector<uint8_t> Ciphertext::Serialize(const PublicKey& pk) const {
+  if (c_.empty()) throw_msg("Ciphertext::Serialize: empty ciphertext");
+  const auto m = mont_words(c_, pk.n_square_, true);
+  return wire_bytes([&](uint8_t* o, size_t cap, size_t* len) {
+    return heu_wire_put_ciphertext(m.data(), (uint32_t)m.size(), o, cap, len);
+  });
+}
+
+Ciphertext Ciphertext::Deserialize(const PublicKey& pk, std::span<const uint8_t> in) {
+  const auto buf = with_index(in);
+  std::vector<uint32_t> m(pk.ct_words());
+  ck(heu_wire_get_ciphertext(buf.data(), buf.size(), m.data(), (uint32_t)m.size()));
+  if (BigInt::FromWords(m.data(), m.size(), false).Compare(pk.n_square_) >= 0)
+    throw_msg("Ciphertext::Deserialize: value is not below n^2");
+  return Ciphertext(mont_words(m, pk.n_square_, false));
+}
+
+std::vector<uint8_t> PublicKey::Serialize() const {
+  const size_t nw = (n_.BitCount() + 31) / 32;
+  const auto n = big_words(n_, nw), h = big_words(h_s_, 2 * nw);
+  return wire_bytes([&](uint8_t* o, size_t cap, size_t* len) {
+    return heu_wire_put_public_key(n.data(), (uint32_t)nw, h.data(), o, cap, len);
+  });
+}
+
+void PublicKey::Deserialize(std::span<const uint8_t> in) {
+  const auto buf = with_index(in);
+  uint32_t nw = 0;
+  ck(heu_wire_get_public_key(buf.data(), buf.size(), nullptr, nullptr, 0, &nw));
+  std::vector<uint32_t> n(nw), h(2 * (size_t)nw);
+  ck(heu_wire_get_public_key(buf.data(), buf.size(), n.data(), h.data(), nw, &nw));
+  *this = PublicKey();
+  n_ = BigInt::FromWords(n.data(), n.size(), false);
+  h_s_ = BigInt::FromWords(h.data(), h.size(), false);
+  Init();
+}
+
+std::vector<uint8_t> SecretKey::Serialize() const {
+  const size_t pw = (std::max(p_.BitCount(), q_.BitCount()) + 31) / 32;
+  const auto p = big_words(p_, pw), q = big_words(q_, pw);
+  return wire_bytes([&](uint8_t* o, size_t cap, size_t* len) {
+    return heu_wire_put_secret_key(p.data(), q.data(), (uint32_t)pw, o, cap, len);
+  });
+}
+
+void SecretKey::Deserialize(std::span<const uint8_t> in) {
+  const auto buf = with_index(in);
+  uint32_t pw = 0;
+  ck(heu_wire_get_secret_key(buf.data(), buf.size(), nullptr, nullptr, 0, &pw));
+  std::vector<uint32_t> p(pw), q(pw);
+  ck(heu_wire_get_secret_key(buf.data(), buf.size(), p.data(), q.data(), pw, &pw));
+  *this = SecretKey();
+  p_ = BigInt::FromWords(p.data(), pw, false);
+  q_ = BigInt::FromWords(q.data(), pw, false);
+  lambda_ = ((p_ - BigInt(1)) * (q_ - BigInt(1))) >> 1;
+  const BigInt n = p_ * q_;
+  bn_check(BN_mod_inverse(mu_.bn(), lambda_.bn(), n.bn(), tls_ctx()) != nullptr, "mod_inverse");
+}
 
 void PublicKey::Init() {
   if (key_size_ == 0) key_size_ = n_.BitCount();
@@ -769,8 +877,8 @@ std::vector<Ciphertext> Encryptor::Encrypt(ConstSpan<Plaintext> pts) const {
   return EncryptWithR(pts, r.data(), dk.r_words);
 }
 
-// encryptor.cc:51-66: the audit string records p, rn = h_s^r and c (hex; the
-// engine's values are Z-space, the reference prints its Montgomery form).
+// encryptor.cc:50-54: the audit string records p, rn = h_s^r and c in hex, rn
+// and c in the Montgomery form the reference prints (R for 64-bit words).
 std::pair<std::vector<Ciphertext>, std::vector<std::string>> Encryptor::EncryptWithAudit(
     ConstSpan<Plaintext> pts) const {
   const DeviceKey& dk = dev(pk_);
@@ -787,7 +895,8 @@ std::pair<std::vector<Ciphertext>, std::vector<std::string>> Encryptor::EncryptW
   }
   for (size_t i = 0; i < cnt; ++i) {
     std::vector<uint32_t> one(rn.begin() + i * dk.ct_words, rn.begin() + (i + 1) * dk.ct_words);
-    audit[i] = "p:" + pts[i]->ToHexString() + ",rn:" + hex_words(one) + ",c:" + cts[i].ToString();
+    audit[i] = "p:" + pts[i]->ToHexString() + ",rn:" + hex_words(mont_words(one, pk_.n_square_, true)) +
+               ",c:" + hex_words(mont_words(cts[i].c_, pk_.n_square_, true));
   }
   return {cts, audit};
 }

@@ -81,6 +81,7 @@ using Span = std::span<T* const>;
 namespace paillier_hip {
 
 using Plaintext = BigInt;
+class PublicKey;
 
 // Z-space value c = (1 + n m) h_s^r mod n^2 as little-endian uint32 limbs.
 class Ciphertext {
@@ -90,6 +91,12 @@ class Ciphertext {
   bool operator==(const Ciphertext& o) const { return c_ == o.c_; }
   bool operator!=(const Ciphertext& o) const { return c_ != o.c_; }
   std::string ToString() const;
+  // paillier_z::Ciphertext's msgpack (MSGPACK_DEFINE(c_), paillier_zahlen/
+  // ciphertext.h:37) with c_ in the Montgomery form the reference stores, c R
+  // mod n^2 for R = 2^(64 ceil(bits(n^2) / 64)).  The key gives n^2, since c_
+  // here is the Z-space value.  HEU's variant wrapper appends the schema index.
+  std::vector<uint8_t> Serialize(const PublicKey& pk) const;
+  static Ciphertext Deserialize(const PublicKey& pk, std::span<const uint8_t> in);
   std::vector<uint32_t> c_;
 };
 
@@ -110,6 +117,10 @@ class PublicKey {
   size_t n_words() const;
   bool operator==(const PublicKey& o) const { return n_ == o.n_ && h_s_ == o.h_s_; }
   std::string ToString() const;
+  // msgpack [n, h_s] (public_key.h:64-76); Deserialize calls Init(), as the
+  // reference's msgpack convert does.
+  std::vector<uint8_t> Serialize() const;
+  void Deserialize(std::span<const uint8_t> in);
 };
 
 class SecretKey {
@@ -120,6 +131,10 @@ class SecretKey {
   void Init(const PublicKey& pk);
   bool operator==(const SecretKey& o) const { return p_ == o.p_ && q_ == o.q_; }
   std::string ToString() const;
+  // msgpack [lambda, mu, p, q] (secret_key.h:64-80); after Deserialize, call
+  // Init(pk) to make the key resident (lambda and mu are checked against p, q).
+  std::vector<uint8_t> Serialize() const;
+  void Deserialize(std::span<const uint8_t> in);
 };
 
 class KeyGenerator {

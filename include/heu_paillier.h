@@ -193,6 +193,50 @@ int heu_prefix_mul(heu_key *key, const uint32_t *ct, const int64_t *offsets, siz
 int heu_to_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
 int heu_from_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
 
+/* ---- wire format (host buffers; no device work) ------------------------ */
+/* HEU's serialized form of ZPaillier objects, so engine ciphertexts and keys
+ * interoperate with HEU peers.  Layouts follow the reference's writers:
+ * paillier_z::Ciphertext is MSGPACK_DEFINE(c_), a msgpack array holding c_
+ * (paillier_zahlen/ciphertext.h:37); PublicKey packs [n, h_s]
+ * (public_key.h:64-76) and SecretKey [lambda, mu, p, q] (secret_key.h:64-80);
+ * phe's variant wrapper appends the schema index as an 8-byte little-endian
+ * size_t (phe/base/serializable_types.cc:50-70), HEU_SCHEMA_ZPAILLIER
+ * (phe/base/schema.h:49).  c_ is the Montgomery form that heu_to_mont writes
+ * (word_bits 64 for yacl's OpenSSL and GMP backends).  A BigInt is a msgpack
+ * str of one sign byte (0 or 1) and the little-endian magnitude without high
+ * zero bytes: yacl's own BigInt packing is not part of the reference, so that
+ * inner byte layout is this engine's.  Writers return HEU_OK with the length
+ * in *len, or HEU_ERANGE with the length needed when cap is too small (out
+ * may then be NULL).  Readers check the layout, the schema index and that
+ * values fit the given limbs (HEU_EINVAL / HEU_ERANGE otherwise). */
+#define HEU_SCHEMA_ZPAILLIER 4
+/* phe::Ciphertext::Serialize: `mont` holds words limbs of c R mod n^2. */
+int heu_wire_put_ciphertext(const uint32_t *mont, uint32_t words, uint8_t *out, size_t cap, size_t *len);
+int heu_wire_get_ciphertext(const uint8_t *in, size_t len, uint32_t *mont, uint32_t words);
+/* numpy DenseMatrix<Ciphertext>::Serialize (numpy/matrix.h:221-265):
+ * [rows, cols, ndim, [items]], every item the bytes of heu_wire_put_ciphertext
+ * as a msgpack str.  mont: rows * cols values in column-major order (Eigen's
+ * storage order).  The reader gives the shape (rows, cols, ndim may be NULL)
+ * and, with mont != NULL, the values (cap_values >= rows * cols). */
+int heu_wire_put_matrix(const uint32_t *mont, int64_t rows, int64_t cols, int64_t ndim, uint32_t words,
+                        uint8_t *out, size_t cap, size_t *len);
+int heu_wire_get_matrix(const uint8_t *in, size_t len, int64_t *rows, int64_t *cols, int64_t *ndim, uint32_t *mont,
+                        uint32_t words, size_t cap_values);
+/* phe::PublicKey / phe::SecretKey for ZPaillier.  The secret key's lambda =
+ * (p-1)(q-1)/2 and mu = lambda^-1 mod n (key_generator.cc:26-60) are computed
+ * from p and q when written and checked against them when read.  Readers
+ * report the word count (*n_words = words of n; *prime_words = words of the
+ * larger prime) and write the values when the buffers are given (h_s gets
+ * 2 * cap_words limbs). */
+int heu_wire_put_public_key(const uint32_t *n, uint32_t n_words, const uint32_t *h_s, uint8_t *out, size_t cap,
+                            size_t *len);
+int heu_wire_get_public_key(const uint8_t *in, size_t len, uint32_t *n, uint32_t *h_s, uint32_t cap_words,
+                            uint32_t *n_words);
+int heu_wire_put_secret_key(const uint32_t *p, const uint32_t *q, uint32_t prime_words, uint8_t *out, size_t cap,
+                            size_t *len);
+int heu_wire_get_secret_key(const uint8_t *in, size_t len, uint32_t *p, uint32_t *q, uint32_t cap_words,
+                            uint32_t *prime_words);
+
 /* ---- multi-task batches ------------------------------------------------ */
 /* A batch runs many tasks -- vectors of any length, on any mix of resident
  * 2048- and 3072-bit keys -- as one dispatch.  The elementwise work of every

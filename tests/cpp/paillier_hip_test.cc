@@ -237,6 +237,14 @@ void PlaintextEvaluate(const Keys& k) {
   EXPECT_FALSE(below.FitsInt64());
 }
 
+// R mod n^2 of the Montgomery form the reference stores and prints, for
+// yacl's 64-bit-word backends: R = 2^(64 ceil(bits(n^2) / 64)).
+BigInt mont_r(const BigInt& n2) {
+  BigInt R;
+  BN_set_bit(R.bn(), (int)(64 * ((n2.BitCount() + 63) / 64)));
+  return nnmod(R, n2);
+}
+
 // paillier_test.cc:188-233 (BigNumberTest EncDec / SubTest over the listed values)
 void BigNumberEncDecSub(const Keys& k) {
   Encryptor encryptor(k.pk);
@@ -252,8 +260,10 @@ void BigNumberEncDecSub(const Keys& k) {
     Ciphertext xe = encryptor.Encrypt(BigInt(x));
     auto xa = encryptor.EncryptWithAudit(BigInt(x));
     EXPECT_EQ(decryptor.Decrypt(xa.first), BigInt(x));
+    const BigInt n2 = k.pk.n_ * k.pk.n_;
+    const std::string c_hex = mod_mul(ct_value(xa.first), mont_r(n2), n2).ToHexString();
     EXPECT_TRUE(xa.second.rfind("p:", 0) == 0 && xa.second.find(",rn:") != std::string::npos &&
-                xa.second.find(",c:" + xa.first.ToString()) != std::string::npos);
+                xa.second.find(",c:" + c_hex) != std::string::npos);
     evaluator.SubInplace(&xe, BigInt(share_a));
     BigInt raw = decryptor.Decrypt(xe);
     // share_a + share_b == x in int64 arithmetic (wrapping, as the reference's Get<int64_t>)
@@ -504,6 +514,58 @@ void SpanRates(const Keys& k, size_t count) {
               count, count / te, count / ta, count / tm, count / td);
 }
 
+// The backend objects' serialized form: msgpack of paillier_z::Ciphertext
+// (ciphertext.h:37, c_ in Montgomery form), PublicKey [n, h_s] and SecretKey
+// [lambda, mu, p, q] (public_key.h:64-76, secret_key.h:64-80); keys and
+// ciphertexts re-created from the bytes decrypt as before, and the audit
+// strings carry the Montgomery forms (encryptor.cc:50-54).
+BigInt str_value(const std::vector<uint8_t>& b, size_t at) {
+  size_t len = 0, off = 0;
+  if (b[at] >= 0xa0 && b[at] <= 0xbf) len = b[at] & 0x1f, off = at + 1;
+  else if (b[at] == 0xd9) len = b[at + 1], off = at + 2;
+  else if (b[at] == 0xda) len = ((size_t)b[at + 1] << 8) | b[at + 2], off = at + 3;
+  EXPECT_TRUE(len >= 1 && b[off] == 0);  // sign byte: non-negative
+  std::vector<uint8_t> mag(b.begin() + off + 1, b.begin() + off + len);
+  mag.resize((mag.size() + 3) / 4 * 4 + 4, 0);
+  return BigInt::FromWords(reinterpret_cast<const uint32_t*>(mag.data()), mag.size() / 4, false);
+}
+
+void Serialization(const Keys& k) {
+  const BigInt n2 = k.pk.n_ * k.pk.n_;
+  const BigInt R = mont_r(n2);
+  PublicKey pk2;
+  pk2.Deserialize(k.pk.Serialize());
+  EXPECT_TRUE(pk2 == k.pk);
+  const auto skb = k.sk.Serialize();
+  SecretKey sk2;
+  sk2.Deserialize(skb);
+  sk2.Init(pk2);
+  EXPECT_TRUE(sk2 == k.sk);
+  EXPECT_TRUE(sk2.lambda_ == k.sk.lambda_ && sk2.mu_ == k.sk.mu_);
+  EXPECT_EQ(skb[0], 0x94);  // fixarray(4)
+  Encryptor enc(k.pk);
+  Decryptor dec2(pk2, sk2);
+  std::vector<uint8_t> last;
+  for (int64_t m : {(int64_t)0, (int64_t)1, (int64_t)-1, (int64_t)123456789, std::numeric_limits<int64_t>::min(),
+                    std::numeric_limits<int64_t>::max()}) {
+    const Ciphertext c = enc.Encrypt(BigInt(m));
+    const auto b = c.Serialize(k.pk);
+    EXPECT_EQ(b[0], 0x91);  // fixarray(1): MSGPACK_DEFINE(c_)
+    EXPECT_TRUE(str_value(b, 1) == mod_mul(ct_value(c), R, n2));
+    const Ciphertext c2 = Ciphertext::Deserialize(pk2, b);
+    EXPECT_TRUE(c2 == c);
+    EXPECT_TRUE(dec2.Decrypt(c2) == BigInt(m));
+    last = b;
+  }
+  last.pop_back();
+  EXPECT_THROW(Ciphertext::Deserialize(pk2, last));
+  const auto audit = enc.EncryptWithAudit(BigInt(-77));
+  const std::string want_c = "c:" + mod_mul(ct_value(audit.first), R, n2).ToHexString();
+  EXPECT_TRUE(audit.second.rfind("p:-4D,rn:", 0) == 0);
+  EXPECT_TRUE(audit.second.size() > want_c.size() &&
+              audit.second.compare(audit.second.size() - want_c.size(), want_c.size(), want_c) == 0);
+}
+
 int main() {
   Keys k2048, k3072, seeded;
   run("KeyGenerator.Generate(2048)", [&] {
@@ -530,6 +592,8 @@ int main() {
   run("Concurrent chunks coalesce (2048 + 3072 keys)", [&] { ConcurrentChunks(seeded, k3072); });
   run("Vectorized.SpanRates(2048, 65536)", [&] { SpanRates(seeded, 65536); });
   run("Decryptor rejects an unpaired key", [&] { EXPECT_THROW(Decryptor(k2048.pk, k3072.sk)); });
+  run("Serialization(2048)", [&] { Serialization(k2048); });
+  run("Serialization(3072)", [&] { Serialization(k3072); });
   if (g_failures) {
     std::printf("%d FAILURE(S)\n", g_failures);
     return 1;
