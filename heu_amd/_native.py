@@ -122,9 +122,9 @@ def check(rc: int) -> None:
 
 class HeuServerConfig(ctypes.Structure):
     _fields_ = [("workspace_bytes", SZ), ("max_inflight", U32), ("max_batch_tasks", U32),
-                ("batch_rounds", U32), ("parts_per_round", U32), ("urgent_ms", U32)]
+                ("batch_rounds", U32), ("parts_per_round", U32), ("urgent_ms", U32), ("max_workspace_bytes", SZ)]
 
 
 class HeuServerStats(ctypes.Structure):
     _fields_ = [("submitted", U64), ("completed", U64), ("failed", U64), ("batches", U64),
-                ("max_batch_requests", U64), ("peak_workspace_bytes", SZ)]
+                ("max_batch_requests", U64), ("peak_workspace_bytes", SZ), ("workspace_bytes", SZ)]

@@ -19,8 +19,9 @@
 //     every request, until the room is used (a sum or dot is taken whole); an
 //     urgent request at the head of the order takes all the room it can use;
 //   - trim to the longest prefix whose plan fits the largest free block of the
-//     workspace arena, plan it into that block, copy its work lists up from
-//     pinned staging and enqueue it on the slot's stream.
+//     workspace arena (first growing the arena by a new device segment while
+//     hipMemGetInfo shows HBM to spare), plan it into that block, copy its work
+//     lists up from pinned staging and enqueue it on the slot's stream.
 // A long request therefore advances a slice per batch while short requests
 // that arrive meanwhile join the next batch; it completes when its last
 // element does.
@@ -29,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -67,6 +69,7 @@ struct Request {
   long whole_units = 0;
   long n = 0, admitted = 0, finished = 0;         // elements (1 for a whole request)
   long inflight = 0;                              // parts admitted to a slot and not yet retired
+  double elem_cost = 1.0;                         // Montgomery products mod n^2 per element (whole: per request)
   size_t sa = 0, sb = 0, so = 0;                  // bytes per element of a, b and out
   Clock::time_point t_submit, t_done;
   int state = kQueued;
@@ -74,6 +77,11 @@ struct Request {
 
   long units_for(long elems) const { return whole ? whole_units : (elems + gpu - 1) / gpu * factor; }
   long remaining_units() const { return whole ? (admitted ? 0 : whole_units) : units_for(n - admitted); }
+  // Work left, in Montgomery products mod a 2048-bit key's n^2: what orders
+  // requests.  Work units differ in cost by three orders of magnitude between
+  // ops (a round of ct+ct units takes ~0.1 ms, one of decryption units ~30 ms),
+  // so units or rounds would rank a long cheap vector behind short expensive ones.
+  double remaining_cost() const { return whole ? (admitted ? 0.0 : elem_cost) : (double)(n - admitted) * elem_cost; }
   heu_task part(long begin, long count) const {
     if (whole) return task;
     heu_task p = task;
@@ -85,10 +93,33 @@ struct Request {
   }
 };
 
+// Montgomery products mod n^2 per element of each op, for the key's table
+// window: encryption and Randomize take one table product per window and one
+// more; ct+ct and ct+pt two; ct x int32 about 46 (30 squarings, 16 products)
+// with the inversion of negative scalars amortized; a dot term that and one
+// more; a sum term one.  A decryption runs two exponentiations of about 1210
+// products mod p^2, each a quarter of one mod n^2.
+double op_cost(const heu_key* k, int op) {
+  switch (op) {
+    case HEU_OP_ENCRYPT_I64:
+    case HEU_OP_RANDOMIZE: return (double)k->windows + 1.0;
+    case HEU_OP_ADD:
+    case HEU_OP_ADD_I64: return 2.0;
+    case HEU_OP_MUL_I64: return 46.0;
+    case HEU_OP_DOT_I64: return 47.0;
+    case HEU_OP_SUM: return 1.0;
+    case HEU_OP_DECRYPT: return 2.0 * 1210.0 / 4.0;
+    default: return 1.0;
+  }
+}
+
 // Fills in the request's shape and geometry (caller holds ctx->mu).
 void describe(heu_ctx* ctx, Request& r) {
   const heu_task& t = r.task;
   const bool dec = t.op == HEU_OP_DECRYPT;
+  // products mod a larger n^2 cost the square of the size ratio
+  const double size = (double)t.key->ct_words / 128.0;
+  r.elem_cost = op_cost(t.key, t.op) * size * size;
   r.shape = dec ? &t.key->dshape().wl : &t.key->ops->wl_n;
   r.round = std::max<long>(1, worklist_round_units(ctx, r.shape));
   r.gpu = r.shape->groups_per_unit;
@@ -100,6 +131,7 @@ void describe(heu_ctx* ctx, Request& r) {
     const long chunk = reduce_chunk0(ctx, t.key->ops, t.op == HEU_OP_DOT_I64, n);  // level 0 of the reduction
     r.whole_units = (n / chunk + (long)t.segments + r.gpu - 1) / r.gpu;
     r.n = 1;
+    r.elem_cost *= (double)std::max(1L, n);
     return;
   }
   r.n = (long)t.count;
@@ -128,12 +160,30 @@ struct Slot {
   bool drained = false;                           // the stream was synchronized: retire without the event
 };
 
-// First-fit allocator over [0, size) with coalescing.
+// First-fit allocator over device segments (separate allocations) with
+// coalescing.  Offsets are virtual: segment i covers [base_i, base_i + size_i),
+// and a gap between segments keeps free blocks from merging across them, so a
+// block always lies inside one allocation.
 struct Arena {
+  static constexpr size_t kGap = 256;
   std::map<size_t, size_t> free_;  // offset -> bytes
-  void init(size_t size) {
-    free_.clear();
-    free_[0] = size;
+  std::map<size_t, std::unique_ptr<DevBuf>> segs_;  // base -> allocation
+  size_t end_ = 0, capacity = 0, largest_seg = 0;
+  int add(size_t bytes) {
+    auto seg = std::make_unique<DevBuf>();
+    const int rc = seg->alloc(bytes);
+    if (rc) return rc;
+    const size_t base = end_;
+    free_[base] = bytes;
+    segs_[base] = std::move(seg);
+    end_ = base + bytes + kGap;
+    capacity += bytes;
+    largest_seg = std::max(largest_seg, bytes);
+    return HEU_OK;
+  }
+  char* ptr(size_t off) const {
+    auto it = std::prev(segs_.upper_bound(off));
+    return it->second->as<char>() + (off - it->first);
   }
   size_t largest() const {
     size_t m = 0;
@@ -169,6 +219,10 @@ struct Arena {
   }
 };
 
+// Device memory the server leaves free when it grows its workspace: 5% of the
+// device, at least 2 GiB (keys, tables and other users of the device).
+size_t hbm_reserve(size_t total) { return std::max<size_t>((size_t)2 << 30, total / 20); }
+
 size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // One part of a batch under construction.
@@ -182,7 +236,6 @@ struct Pick {
 struct heu_server {
   heu_ctx* ctx = nullptr;
   heu_server_config cfg{};
-  DevBuf ws;
   Arena arena;
   size_t in_use = 0;
   std::mutex mu;
@@ -200,6 +253,7 @@ struct heu_server {
   uint64_t fault_batch = 0;
 
   void run();
+  bool grow(size_t need);
   bool retire();
   bool admit(std::unique_lock<std::mutex>& lk);
   bool select(std::vector<Pick>* picks, const heu_hip::WorklistShape** shape);
@@ -261,7 +315,7 @@ bool heu_server::select(std::vector<Pick>* picks, const heu_hip::WorklistShape**
     if (age_ms > (double)cfg.urgent_ms)
       order.emplace_back(-1.0 - age_ms, t);
     else
-      order.emplace_back((double)r.remaining_units() / (double)r.round / (1.0 + age_ms / kAgingMs), t);
+      order.emplace_back(r.remaining_cost() / (1.0 + age_ms / kAgingMs), t);
   }
   std::stable_sort(order.begin(), order.end(),
                    [](const std::pair<double, uint64_t>& x, const std::pair<double, uint64_t>& y) {
@@ -353,7 +407,7 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   std::vector<heu_task> tasks;
   for (const Pick& p : picks) tasks.push_back(reqs[p.ticket].part(p.begin, p.count));
   const size_t n = tasks.size();
-  const size_t free_bytes = arena.largest(), total = ws.bytes;
+  size_t free_bytes = arena.largest();
   lk.unlock();
   // longest admissible prefix: its plan fits the largest free block
   auto size_of = [&](size_t k, PlanOut* p) {
@@ -364,6 +418,13 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   size_t k = n;
   PlanOut p;
   int rc = size_of(k, &p);
+  if (rc == HEU_OK && !fits(p)) {
+    // admission by free device memory: a batch that does not fit grows the
+    // workspace while the device has HBM to spare
+    lk.lock();
+    if (grow(round256(p.bytes))) free_bytes = arena.largest();
+    lk.unlock();
+  }
   if (rc == HEU_OK && !fits(p)) {
     size_t lo = 0, hi = n;  // fits(lo) holds (or lo == 0), fits(hi) does not
     while (hi - lo > 1) {
@@ -382,7 +443,8 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
     lk.unlock();
     const int rc1 = size_of(1, &one);
     lk.lock();
-    if (rc1 != HEU_OK || round256(one.bytes) > total) {
+    if (rc1 == HEU_OK && round256(one.bytes) > arena.largest_seg) (void)grow(round256(one.bytes));
+    if (rc1 != HEU_OK || round256(one.bytes) > arena.largest_seg) {
       fail_request(picks[0].ticket,
                    rc1 != HEU_OK ? heu_last_error() : "request needs more workspace than the server has");
       cv_done.notify_all();
@@ -415,7 +477,7 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, k);
   lk.unlock();
   PlanOut real;
-  char* base = ws.as<char>() + off;
+  char* base = arena.ptr(off);
   {
     std::lock_guard<std::mutex> cl(ctx->mu);
     rc = plan_batch(ctx, tasks.data(), k, base, &real);
@@ -462,6 +524,24 @@ bool heu_server::admit(std::unique_lock<std::mutex>& lk) {
   return true;
 }
 
+// Adds a workspace segment of at least `need` bytes (called with mu held):
+// half the initial workspace when the device can spare it, else exactly
+// `need`, and only while hipMemGetInfo leaves hbm_reserve() free and the
+// workspace stays within max_workspace_bytes.
+bool heu_server::grow(size_t need) {
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return false;
+  const size_t reserve = hbm_reserve(totalb);
+  const size_t cap = cfg.max_workspace_bytes;
+  const size_t room = !cap ? SIZE_MAX : (cap > arena.capacity ? (cap - arena.capacity) & ~(size_t)255 : 0);
+  size_t bytes = std::min(round256(std::max(need, cfg.workspace_bytes / 2)), room);
+  if (bytes < need || freeb < reserve + bytes) bytes = need;
+  if (bytes > room || freeb < reserve + bytes) return false;
+  if (arena.add(bytes) != HEU_OK) return false;
+  st.workspace_bytes = arena.capacity;
+  return true;
+}
+
 void heu_server::run() {
   (void)hipSetDevice(ctx->device);
   std::unique_lock<std::mutex> lk(mu);
@@ -504,13 +584,13 @@ int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** o
   if (!srv->cfg.max_batch_tasks) srv->cfg.max_batch_tasks = 512;
   if (!srv->cfg.batch_rounds) srv->cfg.batch_rounds = 1;
   if (!srv->cfg.parts_per_round) srv->cfg.parts_per_round = 8;
-  if (!srv->cfg.urgent_ms) srv->cfg.urgent_ms = 300;
+  if (!srv->cfg.urgent_ms) srv->cfg.urgent_ms = 150;
   if (srv->cfg.max_inflight > 8) return fail(HEU_EINVAL, "max_inflight must be at most 8");
   if (const char* f = std::getenv("HEU_SERVER_FAULT_BATCH")) srv->fault_batch = std::strtoull(f, nullptr, 10);
   HIP_TRY(hipSetDevice(ctx->device));
-  int rc = srv->ws.alloc(round256(srv->cfg.workspace_bytes));
+  int rc = srv->arena.add(round256(srv->cfg.workspace_bytes));
   if (rc) return rc;
-  srv->arena.init(srv->ws.bytes);
+  srv->st.workspace_bytes = srv->arena.capacity;
   srv->slots.resize(srv->cfg.max_inflight);
   for (Slot& s : srv->slots) {
     hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);

@@ -598,11 +598,11 @@ class Server(_TaskOps):
     the server keeps every operand alive until the request is waited."""
 
     def __init__(self, ctx: Context, workspace_bytes: int = 0, max_inflight: int = 0, max_batch_tasks: int = 0,
-                 batch_rounds: int = 0, parts_per_round: int = 0, urgent_ms: int = 0):
+                 batch_rounds: int = 0, parts_per_round: int = 0, urgent_ms: int = 0, max_workspace_bytes: int = 0):
         self.ctx = ctx
         self._keep = {}
         cfg = _native.HeuServerConfig(workspace_bytes, max_inflight, max_batch_tasks, batch_rounds, parts_per_round,
-                                      urgent_ms)
+                                      urgent_ms, max_workspace_bytes)
         h = ctypes.c_void_p()
         check(lib().heu_server_create(ctx._h, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -138,7 +138,7 @@ def test_server_admission_limits(eng, ctx, keys):
     ct = key.encrypt_i64(m, eng.seeded_r(3, 3, len(m), 256))
     big = key.encrypt_i64(np.zeros(200000, np.int64), eng.seeded_r(3, 4, 200000, 256))
     ctx.sync()
-    srv = eng.Server(ctx, workspace_bytes=8 << 20, max_inflight=1)
+    srv = eng.Server(ctx, workspace_bytes=8 << 20, max_inflight=1, max_workspace_bytes=8 << 20)
     t_big, _ = srv.decrypt(key, big, out_words=2)
     t_ok, out = srv.decrypt(key, ct, out_words=2)
     from heu_amd._native import HeuError
@@ -209,4 +209,25 @@ def test_failed_part_waits_for_running_parts(eng, ctx, keys, monkeypatch):
     assert np.array_equal(got[:x].view(np.int64).ravel(), m[:x])
     st = srv.stats()
     assert st["failed"] == 1 and st["batches"] >= 2
+    srv.close()
+
+
+def test_workspace_grows_by_free_hbm(eng, ctx, keys):
+    # admission by free device memory: a request whose batch does not fit the
+    # initial workspace grows it by a new device segment (hipMemGetInfo shows
+    # room), instead of failing; the results stay exact
+    key = keys[0]
+    n = 200000
+    m = (np.arange(n, dtype=np.int64) * 31337) - n
+    ct = key.encrypt_i64(m, eng.seeded_r(14, 1, n, 256))
+    ctx.sync()
+    srv = eng.Server(ctx, workspace_bytes=8 << 20, max_inflight=2)
+    assert srv.stats()["workspace_bytes"] == 8 << 20
+    print("server: decryption needing more than the initial 8 MiB workspace", flush=True)
+    t, out = srv.decrypt(key, ct, out_words=2)
+    srv.wait(t, 120000.0)
+    st = srv.stats()
+    print(f"server workspace grew to {st['workspace_bytes'] >> 20} MiB", flush=True)
+    assert st["workspace_bytes"] > 8 << 20 and st["failed"] == 0
+    assert np.array_equal(host(out).view(np.int64).ravel(), m)
     srv.close()
