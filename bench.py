@@ -80,6 +80,7 @@ def parse():
                     help="also run a 256 MiB device-to-device copy kernel (HBM counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the key's default parameters (automatic table window, 2-lane decryption)")
+    ap.add_argument("--no-spi", action="store_true", help="skip the SPI host-span rates")
     ap.add_argument("--configs", default="3,4,5",
                     help="batched / served configurations to run after the headline one (comma list, '' for none)")
     return ap.parse_args()
@@ -194,6 +195,26 @@ def cpu_baseline(kd, m, r, k):
         "single_core_per_op": single,
         "per_op_3072": r3072["rates"],
     }
+
+
+def spi_span_rates(count: int = 65536) -> dict:
+    """The HEU-facing rate: the vectorized-SPI classes with host spans (host
+    conversions, copies and a sync per call), timed by heu_amd/spi/span_bench
+    beside the same engine calls on device-resident data (r of key_bits/2 bits,
+    as the SPI draws it)."""
+    import subprocess
+
+    from heu_amd import build as hb
+
+    if not os.path.exists(hb.SPAN_BENCH):
+        return {"error": f"{hb.SPAN_BENCH} is not built"}
+    res = subprocess.run([hb.SPAN_BENCH, str(count)], capture_output=True, text=True, timeout=600)
+    if res.returncode != 0:
+        return {"error": (res.stdout + res.stderr)[-400:]}
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    spi, dev = out["spi_host_spans_per_s"], out["engine_device_resident_per_s"]
+    out["host_share"] = {op: 1.0 - spi[op] / dev[op] for op in spi}
+    return out
 
 
 def cpu_work_seconds(work: dict, cpu: dict) -> float:
@@ -456,6 +477,8 @@ def main():
             line[f"served_highest_rate_p99_within_2x_batch_{stat}"] = max(met) if met else None
     if extra:
         line["configs"] = extra
+    if not args.no_spi:
+        line["spi_span"] = spi_span_rates()
     print(json.dumps(line))
 
 

@@ -68,12 +68,13 @@ def build(force: bool = True) -> str:
 
 SPI_LIB = os.path.join(HERE, "libheu_paillier_spi.so")
 SPI_TEST = os.path.join(os.path.dirname(HERE), "tests", "cpp", "build", "paillier_hip_test")
+SPAN_BENCH = os.path.join(HERE, "spi", "build", "span_bench")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
 def build_spi() -> str:
-    """The C++ vectorized-SPI classes (host code over the C ABI) and their test
-    binary; both host-only, so g++ builds them."""
+    """The C++ vectorized-SPI classes (host code over the C ABI), their test
+    binary and the span-rate program; all host-only, so g++ builds them."""
     include = os.path.join(os.path.dirname(HERE), "include")
     spi = os.path.join(HERE, "spi")
     common = ["g++", "-std=c++20", "-O2", "-fPIC", "-Wall", "-pthread", f"-I{include}", f"-I{spi}",
@@ -93,6 +94,15 @@ def build_spi() -> str:
     if res.returncode != 0:
         raise RuntimeError(f"SPI test build failed: {' '.join(cmd)}\n{res.stderr}")
     os.replace(SPI_TEST + ".tmp", SPI_TEST)
+    # the span-rate program bench.py runs (SPI with host spans beside device-resident calls)
+    os.makedirs(os.path.dirname(SPAN_BENCH), exist_ok=True)
+    cmd = [*common, "-o", SPAN_BENCH + ".tmp", os.path.join(spi, "span_bench.cc"), f"-L{HERE}", "-lheu_paillier_spi",
+           "-lheu_paillier", f"-L{ROCM}/lib", "-lamdhip64", "-lcrypto", "-Wl,-rpath,$ORIGIN/../..",
+           f"-Wl,-rpath,{ROCM}/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"span bench build failed: {' '.join(cmd)}\n{res.stderr}")
+    os.replace(SPAN_BENCH + ".tmp", SPAN_BENCH)
     return SPI_LIB
 
 
