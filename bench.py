@@ -55,6 +55,8 @@ ALGO_MACS = {
     "pow_i64": 44 * M(128),                 # ct x int32 (algorithmic count: 44 products)
     "dec_half": 2 * (1023 + 171 + 16) * M(64),  # both CRT exponentiations per element
     "reduce_sum": M(128),                   # one product per summed ciphertext
+    "mulmod_m": M(128),                     # ct + ct in engine form
+    "reduce_sum_m": M(128),                 # sum term in engine form
 }
 # Algorithmic HBM bytes per element (inputs read + outputs written; table and
 # window-table reads are reuse traffic, not algorithmic).
@@ -64,6 +66,8 @@ ALGO_BYTES = {
     "pow_i64": 516 + 512,
     "dec_half": 512 + 16,
     "reduce_sum": 512,                      # per input ciphertext
+    "mulmod_m": 3 * 576,                    # engine form: 144 limbs of 4 bytes per value
+    "reduce_sum_m": 576,
 }
 VALU_MAC_PEAK_T = 64 * 256 * 2.4e9 / 1e12   # 39.32 T MAC/s
 HBM_PEAK_GBS = 8000.0
@@ -329,6 +333,22 @@ def main():
     ctx.sync()
     sum_ms, _ = ctx.profile_read("reduce_sum")
     ctx.profile(False)
+    # engine form: the same ct+ct and axis-0 sum on values kept on the device as
+    # Montgomery limbs between calls (one product per element or term)
+    ct_m = key.to_mform(ct_res)
+    key.add_mform(ct_m, ct_m)
+    key.sum_mform(ct_m, offs)
+    ctx.sync()
+    ctx.profile(True)
+    for _ in range(2):
+        key.add_mform(ct_m, ct_m)
+        key.sum_mform(ct_m, offs)
+    ctx.sync()
+    mm_ms, mm_launches = ctx.profile_read("mulmod_m")
+    summ_ms, _ = ctx.profile_read("reduce_sum")
+    ctx.profile(False)
+    del ct_m
+    extra_rows = [("reduce_sum", (sum_ms, 0)), ("mulmod_m", (mm_ms, mm_launches)), ("reduce_sum_m", (summ_ms, 0))]
     kernels = {}
     traffic_all = {}
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -337,8 +357,8 @@ def main():
             traffic_all = json.load(open(tpath))
         except Exception:
             traffic_all = {}
-    for name, (total, launches) in list(kern.items()) + ([("reduce_sum", (sum_ms, 0))] if sum_ms > 0 else []):
-        elems = 2 * N if name == "reduce_sum" else N * args.steps
+    for name, (total, launches) in list(kern.items()) + [r for r in extra_rows if r[1][0] > 0]:
+        elems = 2 * N if name in ("reduce_sum", "mulmod_m", "reduce_sum_m") else N * args.steps
         secs = total / 1e3
         row = {
             "ms_total": total,

@@ -48,6 +48,11 @@ SIGNATURES = {
     "heu_prefix_mul": (I, [P, P, P, SZ, P]),
     "heu_to_mont": (I, [P, P, SZ, U32, P]),
     "heu_from_mont": (I, [P, P, SZ, U32, P]),
+    "heu_key_mform_words": (I, [P, ctypes.POINTER(U32)]),
+    "heu_to_mform": (I, [P, P, SZ, P]),
+    "heu_from_mform": (I, [P, P, SZ, P]),
+    "heu_add_mform": (I, [P, P, P, SZ, P]),
+    "heu_sum_mform": (I, [P, P, P, SZ, P]),
     "heu_batch_create": (I, [P, P, SZ, U32, ctypes.POINTER(P)]),
     "heu_batch_run": (I, [P]),
     "heu_batch_destroy": (I, [P]),

@@ -221,7 +221,7 @@ void plan_segmented(Builder& b, heu_key* key, const heu_task& t, WorkList& wl) {
         o->reduce_mont(st, mod, pneg, ch, nch, cneg);
       }, 2);
     } else {
-      b.then([=](hipStream_t st) { o->reduce_sum(st, mod, false, ppos, ch, nch, cpos, nullptr); });
+      b.then([=](hipStream_t st) { o->reduce_sum(st, mod, false, ppos, ch, nch, cpos, nullptr, nullptr); });
     }
   }
   // the last level holds one partial per segment, in segment order

@@ -404,15 +404,18 @@ int upload_meta(heu_ctx* ctx, Scratch& sc, size_t off, const void* host, size_t 
 
 // Generic segmented reduction.  dot: weights given (device), else plain product.
 // With `fixed` set, a plain product carves its scratch from that buffer only.
+// With `mform`, a plain product reads and writes engine-form values.
 int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t* offsets, size_t segments,
-              uint32_t* out, const int64_t* idx = nullptr, char* fixed = nullptr, size_t fixed_bytes = 0) {
+              uint32_t* out, const int64_t* idx = nullptr, char* fixed = nullptr, size_t fixed_bytes = 0,
+              bool mform = false) {
   heu_ctx* ctx = key->ctx;
   const Ops* o = key->ops;
   const ModArgs mod = key->mod_n2();
   const size_t NLB = (size_t)key->nl_n * 4;
   const bool dot = w != nullptr;
   if (segments == 0) return HEU_OK;
-  if (fixed && dot) return fail(HEU_EINVAL, "internal: a dot product needs the context's scratch pool");
+  if ((fixed || mform) && dot) return fail(HEU_EINVAL, "internal: a dot product needs Z-space inputs and the pool");
+  if (mform && idx) return fail(HEU_EINVAL, "internal: engine-form sums take no index list");
   std::vector<long> start(segments), len(segments);
   for (size_t s = 0; s < segments; ++s) {
     start[s] = offsets[s];
@@ -464,7 +467,8 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
     LAUNCH(ctx, "reduce_dot0", o->reduce_dot0(st, mod, ct, w, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off),
                    sc.at<uint32_t>(lv[0].neg_off), sc.at<uint32_t>(tab_off), idx));
   else
-    LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, true, ct, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(), sc.at<uint32_t>(lv[0].pos_off), idx));
+    LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, !mform, ct, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(),
+                                            sc.at<uint32_t>(lv[0].pos_off), idx, mform ? mod.one : nullptr));
   for (size_t i = 1; i < lv.size(); ++i) {
     if (dot) {
       LAUNCH(ctx, "reduce_mont", o->reduce_mont(st, mod, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off), (long)lv[i].ch.size(),
@@ -473,11 +477,16 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
                      sc.at<uint32_t>(lv[i].neg_off)));
     } else {
       LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, false, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off),
-                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off), nullptr));
+                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off), nullptr, nullptr));
     }
   }
   // the last level has one chunk per segment, in segment order
   const Level& top = lv.back();
+  if (mform) {  // a product of engine-form values is in engine form: done
+    HIP_TRY(hipMemcpyAsync(out, sc.at<char>(top.pos_off), segments * NLB, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipGetLastError());
+    return HEU_OK;
+  }
   HIP_TRY(hipMemcpyAsync(sc.at<char>(top_pos), sc.at<char>(top.pos_off), segments * NLB, hipMemcpyDeviceToDevice, st));
   if (!dot) {
     std::vector<int64_t> nt(len.begin(), len.end());
@@ -1024,6 +1033,47 @@ int heu_to_mont(heu_key* key, const uint32_t* ct, size_t count, uint32_t word_bi
 
 int heu_from_mont(heu_key* key, const uint32_t* ct, size_t count, uint32_t word_bits, uint32_t* out) {
   return mont_convert(key, ct, count, word_bits, out, false);
+}
+
+int heu_key_mform_words(const heu_key* key, uint32_t* words) {
+  if (!key || !words) return fail(HEU_EINVAL, "null argument");
+  *words = (uint32_t)key->nl_n;
+  return HEU_OK;
+}
+
+int heu_to_mform(heu_key* key, const uint32_t* ct, size_t count, uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (count == 0) return HEU_OK;
+  if (!ct || !out) return fail(HEU_EINVAL, "null argument");
+  LAUNCH(ctx, "to_mform", o->to_mform(ctx->stream, key->mod_n2(), ct, (long)count, out));
+  HIP_TRY(hipGetLastError());
+  return HEU_OK;
+}
+
+int heu_from_mform(heu_key* key, const uint32_t* mf, size_t count, uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (count == 0) return HEU_OK;
+  if (!mf || !out) return fail(HEU_EINVAL, "null argument");
+  // x R -> x, canonical words: the Montgomery-input finish of a segment product
+  LAUNCH(ctx, "from_mform", o->seg_finish(ctx->stream, key->mod_n2(), mf, nullptr, (long)count, 1, out));
+  HIP_TRY(hipGetLastError());
+  return HEU_OK;
+}
+
+int heu_add_mform(heu_key* key, const uint32_t* a, const uint32_t* b, size_t count, uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (count == 0) return HEU_OK;
+  if (!a || !b || !out) return fail(HEU_EINVAL, "null argument");
+  LAUNCH(ctx, "mulmod_m", o->mulmod_m(ctx->stream, key->mod_n2(), a, b, (long)count, out));
+  HIP_TRY(hipGetLastError());
+  return HEU_OK;
+}
+
+int heu_sum_mform(heu_key* key, const uint32_t* mf, const int64_t* offsets, size_t segments, uint32_t* out) {
+  KEY_PROLOGUE(key);
+  if (!offsets || (segments && !out) || (!mf && segments && offsets[segments] > offsets[0]))
+    return fail(HEU_EINVAL, "null argument");
+  return segmented(key, mf, nullptr, offsets, segments, out, nullptr, nullptr, 0, true);
 }
 
 }  // extern "C"

@@ -90,12 +90,15 @@ typedef void (*EncBigFn)(hipStream_t, const PubArgs&, const uint32_t*, int, cons
 typedef void (*RandFn)(hipStream_t, const PubArgs&, const uint32_t*, const uint32_t*, int, long, uint32_t*,
                        uint32_t*);
 typedef void (*MulModFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
+typedef void (*ConvFn)(hipStream_t, const ModArgs&, const uint32_t*, long, uint32_t*);
 typedef void (*AddPtFn)(hipStream_t, const PubArgs&, const uint32_t*, const int64_t*, long, uint32_t*);
 typedef void (*PowFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, uint32_t*, u8*, uint32_t*);
 typedef void (*PowWordsFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, const u8*, int, long,
                            uint32_t*, u8*, uint32_t*);
+// zin: Z-space words at level 0 (else R29 limbs); mont_one (limbs only): the
+// value of an empty chunk is R mod m instead of 1 (engine-form sums)
 typedef void (*ReduceSumFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const Chunk*, long, uint32_t*,
-                            const int64_t*);
+                            const int64_t*, const uint32_t*);
 typedef void (*ReduceDot0Fn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, const Chunk*, long,
                              uint32_t*, uint32_t*, uint32_t*, const int64_t*);
 typedef void (*PrefixMulFn)(hipStream_t, const ModArgs&, const uint32_t*, const Chunk*, long, uint32_t*);
@@ -153,6 +156,8 @@ struct Ops {
   EncBigFn encrypt_big;
   RandFn randomize;
   MulModFn mulmod;
+  ConvFn to_mform;    // Z-space words -> engine form (k_to_mform)
+  MulModFn mulmod_m;  // engine-form ct (+) ct
   AddPtFn add_pt_i64;
   PowFn pow_i64;
   PowWordsFn pow_words;

@@ -391,6 +391,42 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_add_pt_i64(P
 }
 
 // ---------------------------------------------------------------------------
+// Engine form: ciphertexts kept on the device between calls as R29 Montgomery
+// limbs (interleaved, NL words per element, c R mod n^2, lazy: below 2 n^2).
+// ct (+) ct is then one Montgomery product, (a R)(b R) R^-1 = ab R, read and
+// written as limbs with no word conversion; a product of N such values over
+// N - 1 products is again in engine form.
+// ---------------------------------------------------------------------------
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_to_mform(ModArgs mod, const u32* __restrict__ ct,
+                                                                           long count, u32* __restrict__ out) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  u32 x[K];
+  g.load_words(x, ct + e * W, W);
+  g.set(g.A, x);
+  g.mul_global(mod.r2);  // c R
+  g.save(out + e * NL, g.A);
+}
+
+template <int L, int K, int W>
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_mulmod_m(ModArgs mod, const u32* __restrict__ a,
+                                                                           const u32* __restrict__ b, long count,
+                                                                           u32* __restrict__ out) {
+  constexpr int NL = L * K;
+  LDS_DECL(L, K, W);
+  long e;
+  auto g = grp_setup<L, K, W>(lds, mod, e);
+  if (e >= count) return;
+  g.set_global(g.A, a + e * NL);
+  g.mul_global(b + e * NL);
+  g.save(out + e * NL, g.A);
+}
+
+// ---------------------------------------------------------------------------
 // Powers with 64-bit exponents, fixed 3-bit windows in Montgomery space.
 // T (global scratch, room for 2^kPowWin entries of NL words) receives c^d R
 // for d < 8.  A holds c R on entry and c^mag R on return.  Leading zero
@@ -759,13 +795,18 @@ __global__ __launch_bounds__(kBlock) void k_dec_combine(CombArgs C, const u32* _
 // (gathered sums: selections, bucket sums), else element j.
 __device__ __forceinline__ long at_pos(const i64* __restrict__ idx, long j) { return idx ? (long)idx[j] : j; }
 
+// An empty chunk gives 1; in engine form (mont_one != null, the R29 Montgomery
+// form of 1) it gives R mod m, so that a sum of engine-form values stays in
+// engine form.
 template <int L, int K, int W, bool ZIN>
 __device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const u32* __restrict__ in, const Chunk ch,
-                                                u32* __restrict__ part, const i64* __restrict__ idx = nullptr) {
+                                                u32* __restrict__ part, const i64* __restrict__ idx = nullptr,
+                                                const u32* __restrict__ mont_one = nullptr) {
   constexpr int NL = L * K;
   u32 x[K];
   if (ch.len == 0) {
-    one_limbs<L, K>(x, g.lid);
+    if (mont_one) load_r29<L, K>(x, mont_one, g.lid);
+    else one_limbs<L, K>(x, g.lid);
     store_r29<L, K>(part, x, g.lid);
     return;
   }
@@ -784,12 +825,13 @@ template <int L, int K, int W, bool ZIN>
 __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_sum(ModArgs mod, const u32* __restrict__ in,
                                                                              const Chunk* __restrict__ chunks,
                                                                              long nchunks, u32* __restrict__ part,
-                                                                             const i64* __restrict__ idx) {
+                                                                             const i64* __restrict__ idx,
+                                                                             const u32* __restrict__ mont_one) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e < nchunks) reduce_sum_body<L, K, W, ZIN>(g, in, chunks[e], part + e * NL, idx);
+  if (e < nchunks) reduce_sum_body<L, K, W, ZIN>(g, in, chunks[e], part + e * NL, idx, mont_one);
 }
 
 template <int L, int K, int W>
@@ -1589,9 +1631,15 @@ struct Launch {
     HEU_LAUNCH((k_pow_words<L, K, W>), n, L, st, mod, ct, kmag, kneg, kw, n, out, neg, tab);
   }
   static void reduce_sum(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const Chunk* chunks, long nch,
-                         u32* part, const i64* idx) {
-    if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part, idx);
-    else HEU_LAUNCH((k_reduce_sum<L, K, W, false>), nch, L, st, mod, in, chunks, nch, part, nullptr);
+                         u32* part, const i64* idx, const u32* mont_one) {
+    if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part, idx, nullptr);
+    else HEU_LAUNCH((k_reduce_sum<L, K, W, false>), nch, L, st, mod, in, chunks, nch, part, nullptr, mont_one);
+  }
+  static void to_mform(hipStream_t st, const ModArgs& mod, const u32* ct, long n, u32* out) {
+    HEU_LAUNCH((k_to_mform<L, K, W>), n, L, st, mod, ct, n, out);
+  }
+  static void mulmod_m(hipStream_t st, const ModArgs& mod, const u32* a, const u32* b, long n, u32* out) {
+    HEU_LAUNCH((k_mulmod_m<L, K, W>), n, L, st, mod, a, b, n, out);
   }
   static void reduce_dot0(hipStream_t st, const ModArgs& mod, const u32* in, const i64* w, const Chunk* chunks,
                           long nch, u32* pp, u32* pn, u32* tab, const i64* idx) {
@@ -1707,6 +1755,8 @@ static Ops make_ops(const WorklistShape& wl_n, const DecShape& dec0, const DecSh
   o.encrypt_big = Launch<SN>::encrypt_big;
   o.randomize = Launch<SN>::randomize;
   o.mulmod = Launch<SN>::mulmod;
+  o.to_mform = Launch<SN>::to_mform;
+  o.mulmod_m = Launch<SN>::mulmod_m;
   o.add_pt_i64 = Launch<SN>::add_pt_i64;
   o.pow_i64 = Launch<SN>::pow_i64;
   o.pow_words = Launch<SN>::pow_words;

@@ -445,6 +445,50 @@ class Key:
         check(lib().heu_prefix_mul(self._h, _ptr(ct), _ptr(offs), len(offs) - 1, _ptr(out)))
         return out
 
+    # ------------------------------------------------------------- engine form
+    @property
+    def mform_words(self) -> int:
+        w = ctypes.c_uint32()
+        check(lib().heu_key_mform_words(self._h, ctypes.byref(w)))
+        return w.value
+
+    def _mform_rows(self, t: torch.Tensor) -> int:
+        if t.dim() != 2 or t.shape[1] != self.mform_words:
+            raise ValueError(f"engine-form values must have shape (N, {self.mform_words})")
+        return t.shape[0]
+
+    def to_mform(self, ct) -> torch.Tensor:
+        """Z-space ciphertexts -> engine form (R29 Montgomery limbs, lazy)."""
+        ct = self._dev(ct, torch.int32)
+        n = self._rows(ct)
+        out = torch.empty((n, self.mform_words), dtype=torch.int32, device=self.ctx.device)
+        check(lib().heu_to_mform(self._h, _ptr(ct), n, _ptr(out)))
+        return out
+
+    def from_mform(self, mf) -> torch.Tensor:
+        mf = self._dev(mf, torch.int32)
+        n = self._mform_rows(mf)
+        out = self._ct_out(n)
+        check(lib().heu_from_mform(self._h, _ptr(mf), n, _ptr(out)))
+        return out
+
+    def add_mform(self, a, b) -> torch.Tensor:
+        a, b = self._dev(a, torch.int32), self._dev(b, torch.int32)
+        n = self._mform_rows(a)
+        if self._mform_rows(b) != n:
+            raise ValueError("engine-form operands must have the same number of rows")
+        out = torch.empty_like(a)
+        check(lib().heu_add_mform(self._h, _ptr(a), _ptr(b), n, _ptr(out)))
+        return out
+
+    def sum_mform(self, mf, offsets) -> torch.Tensor:
+        mf = self._dev(mf, torch.int32)
+        offs = self._segments(offsets, self._mform_rows(mf))
+        segs = len(offs) - 1
+        out = torch.empty((segs, self.mform_words), dtype=torch.int32, device=self.ctx.device)
+        check(lib().heu_sum_mform(self._h, _ptr(mf), _ptr(offs), segs, _ptr(out)))
+        return out
+
     def to_mont(self, ct, word_bits: int = 64) -> torch.Tensor:
         ct = self._dev(ct, torch.int32)
         n = self._rows(ct)

@@ -193,6 +193,26 @@ int heu_prefix_mul(heu_key *key, const uint32_t *ct, const int64_t *offsets, siz
 int heu_to_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
 int heu_from_mont(heu_key *key, const uint32_t *ct, size_t count, uint32_t word_bits, uint32_t *out);
 
+/* ---- engine form: ciphertexts resident in Montgomery form --------------- */
+/* Callers that chain operations on the device may keep ciphertexts in the
+ * engine's own form between calls: c R mod n^2 as 29-bit limbs in the
+ * interleaved lane layout, *words (heu_key_mform_words) uint32 words per
+ * element, lazily reduced (below 2 n^2, so bytes of equal values may differ;
+ * heu_from_mform gives the canonical Z-space words).  In that form ct (+) ct
+ * is one Montgomery product per element and a segment sum one per term,
+ * where Z-space I/O costs a second product (or word conversions) per
+ * element.  No counterpart in the reference, whose paillier_z::Ciphertext
+ * also stays in a Montgomery space between operations (c_, encryptor.cc:46,
+ * evaluator.cc:30-37) but with yacl's R. */
+int heu_key_mform_words(const heu_key *key, uint32_t *words);
+int heu_to_mform(heu_key *key, const uint32_t *ct, size_t count, uint32_t *out);
+int heu_from_mform(heu_key *key, const uint32_t *mf, size_t count, uint32_t *out);
+/* Evaluator::Add(ct, ct) (evaluator.cc:30-37) in engine form. */
+int heu_add_mform(heu_key *key, const uint32_t *a, const uint32_t *b, size_t count, uint32_t *out);
+/* heu_sum in engine form: inputs and the per-segment outputs (1 in engine
+ * form for an empty segment) are engine-form values; HOST offsets. */
+int heu_sum_mform(heu_key *key, const uint32_t *mf, const int64_t *offsets, size_t segments, uint32_t *out);
+
 /* ---- wire format (host buffers; no device work) ------------------------ */
 /* HEU's serialized form of ZPaillier objects, so engine ciphertexts and keys
  * interoperate with HEU peers.  Layouts follow the reference's writers:
