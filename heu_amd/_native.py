@@ -71,6 +71,10 @@ SIGNATURES = {
     "heu_wire_put_matrix": (I, [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, U32, P, SZ, ctypes.POINTER(SZ)]),
     "heu_wire_get_matrix": (I, [P, SZ, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(ctypes.c_int64), P, U32, SZ]),
+    "heu_wire_put_matrix_ic": (I, [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, U32, P, SZ,
+                                   ctypes.POINTER(SZ)]),
+    "heu_wire_get_matrix_ic": (I, [P, SZ, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_int64), P, U32, SZ]),
     "heu_wire_put_public_key": (I, [P, U32, P, P, SZ, ctypes.POINTER(SZ)]),
     "heu_wire_get_public_key": (I, [P, SZ, P, P, U32, ctypes.POINTER(U32)]),
     "heu_wire_put_secret_key": (I, [P, P, U32, P, SZ, ctypes.POINTER(SZ)]),

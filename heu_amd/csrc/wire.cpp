@@ -233,6 +233,126 @@ int to_words(const BIGNUM* v, uint32_t* w, uint32_t words) {
 
 uint32_t words_of(const BIGNUM* v) { return (uint32_t)((BN_num_bits(v) + 31) / 32); }
 
+// ---- interconnection form (numpy/matrix.cc:39-111) -------------------------
+// A protobuf DataExchangeProtocol (package org.interconnection.v2.runtime).
+// Its .proto is part of org_interconnection 0.0.1 (MODULE.bazel:45), which the
+// reference does not vendor, so these numbers are this engine's reading of it,
+// held here alone:
+//   message DataExchangeProtocol { ScalarType scalar_type = 1;
+//     string scalar_type_name = 2; oneof container { NdArray v_ndarray = 4; } }
+//   message NdArray { repeated int64 shape = 1; repeated bytes items = 3; }
+//   SCALAR_TYPE_OBJECT = 12
+// Fields are written in field-number order, shape packed, as protobuf's own
+// serializers do; the reader takes either shape encoding, any field order and
+// skips fields it does not know.
+constexpr uint32_t kIcScalarType = 1, kIcScalarTypeName = 2, kIcNdarray = 4;
+constexpr uint32_t kIcShape = 1, kIcItems = 3;
+constexpr uint64_t kIcScalarObject = 12;
+constexpr const char kIcTypeName[] = "ciphertext";  // matrix.cc:29-30 (Typename<phe::Ciphertext>)
+
+enum : uint32_t { kVarint = 0, kFixed64 = 1, kDelimited = 2, kFixed32 = 5 };
+
+void pb_varint(Out& o, uint64_t v) {
+  while (v >= 0x80) {
+    o.byte((uint8_t)(v | 0x80));
+    v >>= 7;
+  }
+  o.byte((uint8_t)v);
+}
+
+void pb_tag(Out& o, uint32_t field, uint32_t wire) { pb_varint(o, ((uint64_t)field << 3) | wire); }
+
+size_t varint_len(uint64_t v) {
+  size_t n = 1;
+  while (v >= 0x80) {
+    v >>= 7;
+    ++n;
+  }
+  return n;
+}
+
+// Reads protobuf's wire format: varints, tags and length-delimited bodies.
+struct Pb {
+  const uint8_t* p;
+  size_t len, at = 0;
+  bool varint(uint64_t* v) {
+    *v = 0;
+    for (int shift = 0; shift < 70; shift += 7) {
+      if (at >= len) return false;
+      const uint8_t b = p[at++];
+      if (shift == 63 && b > 1) return false;  // beyond 64 bits
+      *v |= (uint64_t)(b & 0x7f) << shift;
+      if (!(b & 0x80)) return true;
+    }
+    return false;
+  }
+  bool tag(uint32_t* field, uint32_t* wire) {
+    uint64_t t;
+    if (!varint(&t) || (t >> 3) == 0 || (t >> 3) > 0x1fffffff) return false;
+    *field = (uint32_t)(t >> 3);
+    *wire = (uint32_t)(t & 7);
+    return true;
+  }
+  bool delimited(const uint8_t** body, size_t* n) {
+    uint64_t l;
+    if (!varint(&l) || l > len - at) return false;
+    *body = p + at;
+    *n = (size_t)l;
+    at += (size_t)l;
+    return true;
+  }
+  bool skip(uint32_t wire) {
+    uint64_t v;
+    const uint8_t* b;
+    size_t n;
+    switch (wire) {
+      case kVarint: return varint(&v);
+      case kFixed64: return len - at >= 8 ? (at += 8, true) : false;
+      case kDelimited: return delimited(&b, &n);
+      case kFixed32: return len - at >= 4 ? (at += 4, true) : false;
+      default: return false;  // groups are not used by proto3 messages
+    }
+  }
+};
+
+struct IcMatrix {
+  std::vector<int64_t> shape;
+  std::vector<std::pair<const uint8_t*, size_t>> items;
+};
+
+// NdArray fields of one v_ndarray body, merged into m (a repeated message
+// field's occurrences merge, so shapes and items append).
+bool ic_ndarray(const uint8_t* b, size_t n, IcMatrix* m) {
+  Pb r{b, n};
+  while (r.at < r.len) {
+    uint32_t f, w;
+    if (!r.tag(&f, &w)) return false;
+    if (f == kIcShape && w == kDelimited) {
+      const uint8_t* body;
+      size_t bn;
+      if (!r.delimited(&body, &bn)) return false;
+      Pb s{body, bn};
+      while (s.at < s.len) {
+        uint64_t v;
+        if (!s.varint(&v)) return false;
+        m->shape.push_back((int64_t)v);
+      }
+    } else if (f == kIcShape && w == kVarint) {
+      uint64_t v;
+      if (!r.varint(&v)) return false;
+      m->shape.push_back((int64_t)v);
+    } else if (f == kIcItems && w == kDelimited) {
+      const uint8_t* body;
+      size_t bn;
+      if (!r.delimited(&body, &bn)) return false;
+      m->items.emplace_back(body, bn);
+    } else if (f == kIcShape || f == kIcItems || !r.skip(w)) {
+      return false;  // a known field with the wrong wire type, or a bad field
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -293,6 +413,92 @@ int heu_wire_get_matrix(const uint8_t* in, size_t len, int64_t* rows, int64_t* c
       return fail(HEU_EINVAL, "matrix item %u is not a serialized ZPaillier ciphertext of at most %u words", i, words);
   }
   if (r.at != len) return fail(HEU_EINVAL, "trailing bytes after the matrix");
+  return HEU_OK;
+}
+
+int heu_wire_put_matrix_ic(const uint32_t* mont, int64_t rows, int64_t cols, int64_t ndim, uint32_t words,
+                           uint8_t* out, size_t cap, size_t* len) {
+  if (!len || words == 0 || rows < 0 || cols < 0 || ndim < 0 || ndim > 2) return fail(HEU_EINVAL, "bad argument");
+  if ((ndim == 0 && (rows != 1 || cols != 1)) || (ndim == 1 && cols != 1))
+    return fail(HEU_EINVAL, "a %lld-d matrix cannot be %lld x %lld", (long long)ndim, (long long)rows,
+                (long long)cols);
+  const int64_t count = rows * cols;
+  if (count > 0 && !mont) return fail(HEU_EINVAL, "null argument");
+  // the NdArray body's length comes first, so count it
+  const int64_t shape[2] = {rows, cols};
+  size_t shape_len = 0;
+  for (int64_t i = 0; i < ndim; ++i) shape_len += varint_len((uint64_t)shape[i]);
+  size_t nd_len = ndim ? 1 + varint_len(shape_len) + shape_len : 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const size_t item = ciphertext_len(mont + (size_t)i * words, words);
+    nd_len += 1 + varint_len(item) + item;
+  }
+  Out o{out, cap};
+  pb_tag(o, kIcScalarType, kVarint);
+  pb_varint(o, kIcScalarObject);
+  pb_tag(o, kIcScalarTypeName, kDelimited);
+  pb_varint(o, sizeof(kIcTypeName) - 1);
+  o.bytes(reinterpret_cast<const uint8_t*>(kIcTypeName), sizeof(kIcTypeName) - 1);
+  pb_tag(o, kIcNdarray, kDelimited);
+  pb_varint(o, nd_len);
+  const size_t body_at = o.n;
+  if (ndim) {
+    pb_tag(o, kIcShape, kDelimited);
+    pb_varint(o, shape_len);
+    for (int64_t i = 0; i < ndim; ++i) pb_varint(o, (uint64_t)shape[i]);
+  }
+  for (int64_t i = 0; i < count; ++i) {
+    const uint32_t* v = mont + (size_t)i * words;
+    pb_tag(o, kIcItems, kDelimited);
+    pb_varint(o, ciphertext_len(v, words));
+    put_ciphertext(o, v, words);
+  }
+  if (o.n - body_at != nd_len) return fail(HEU_EINVAL, "internal: NdArray length mismatch");
+  return finish(o, len);
+}
+
+int heu_wire_get_matrix_ic(const uint8_t* in, size_t len, int64_t* rows, int64_t* cols, int64_t* ndim, uint32_t* mont,
+                           uint32_t words, size_t cap_values) {
+  if (!in && len) return fail(HEU_EINVAL, "null argument");
+  Pb r{in, len};
+  uint64_t scalar_type = 0;
+  bool have_nd = false;
+  IcMatrix m;
+  while (r.at < r.len) {
+    uint32_t f, w;
+    if (!r.tag(&f, &w)) return fail(HEU_EINVAL, "not a protobuf DataExchangeProtocol");
+    bool ok;
+    if (f == kIcScalarType && w == kVarint) {
+      ok = r.varint(&scalar_type);
+    } else if (f == kIcNdarray && w == kDelimited) {
+      const uint8_t* b;
+      size_t n;
+      ok = r.delimited(&b, &n) && ic_ndarray(b, n, &m);
+      have_nd = true;
+    } else {
+      ok = f != kIcScalarType && f != kIcNdarray && r.skip(w);
+    }
+    if (!ok) return fail(HEU_EINVAL, "not a protobuf DataExchangeProtocol (field %u)", f);
+  }
+  if (scalar_type != kIcScalarObject)
+    return fail(HEU_EINVAL, "Buffer format illegal, scalar_type=%llu", (unsigned long long)scalar_type);
+  if (!have_nd) return fail(HEU_EINVAL, "unsupported container type: an NdArray is expected");
+  const size_t nd = m.shape.size();
+  const int64_t nr = nd > 0 ? m.shape[0] : 1, nc = nd > 1 ? m.shape[1] : 1;
+  if (nd > 2 || nr < 0 || nc < 0) return fail(HEU_EINVAL, "a ciphertext matrix has at most 2 non-negative dims");
+  if ((uint64_t)nr * (uint64_t)nc != m.items.size())
+    return fail(HEU_EINVAL, "Pb: shape and len not match (%lld x %lld, %zu items)", (long long)nr, (long long)nc,
+                m.items.size());
+  if (rows) *rows = nr;
+  if (cols) *cols = nc;
+  if (ndim) *ndim = (int64_t)nd;
+  if (!mont) return HEU_OK;
+  if (words == 0) return fail(HEU_EINVAL, "words must be positive");
+  if (cap_values < m.items.size())
+    return fail(HEU_ERANGE, "the matrix holds %zu values; room for %zu", m.items.size(), cap_values);
+  for (size_t i = 0; i < m.items.size(); ++i)
+    if (!get_ciphertext(m.items[i].first, m.items[i].second, mont + i * words, words))
+      return fail(HEU_EINVAL, "matrix item %zu is not a serialized ZPaillier ciphertext of at most %u words", i, words);
   return HEU_OK;
 }
 

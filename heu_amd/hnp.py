@@ -26,9 +26,10 @@ from . import wire
 
 
 class MatrixSerializeFormat:
-    """numpy/matrix.h MatrixSerializeFormat: Best is the msgpack form;
-    Interconnection is the org_interconnection protobuf, whose schema is not in
-    the reference tree and is not built here."""
+    """numpy/matrix.h MatrixSerializeFormat: Best is the msgpack form,
+    Interconnection the org_interconnection protobuf (numpy/matrix.cc:39-111;
+    its .proto is not in the reference tree, so its field numbers are the
+    engine's stated ones)."""
 
     Best = "best"
     Interconnection = "interconnection"
@@ -104,26 +105,31 @@ class CiphertextArray:
         return self._colmajor_of(self.rows(), self.cols())
 
     def serialize(self, format: str = MatrixSerializeFormat.Best) -> bytes:
-        """DenseMatrix<Ciphertext>::Serialize (numpy/matrix.h:221-265): every
-        ciphertext in the Montgomery form the reference stores, column-major."""
-        if format != MatrixSerializeFormat.Best:
-            raise NotImplementedError("the interconnection protobuf schema (org_interconnection) is not built")
+        """DenseMatrix<Ciphertext>::Serialize (numpy/matrix.h:221-265), or
+        Serialize4Ic (numpy/matrix.cc:39-76) with the Interconnection format:
+        every ciphertext in the Montgomery form the reference stores,
+        column-major."""
+        put = {MatrixSerializeFormat.Best: wire.put_matrix, MatrixSerializeFormat.Interconnection: wire.put_matrix_ic}
+        if format not in put:
+            raise ValueError(f"unknown matrix serialize format {format!r}")
         key = self.key
         if self.size == 0:
             mont = np.zeros((0, key.ct_words), np.uint32)
         else:
             mont = key.to_mont(self.data, MONT_WORD_BITS).cpu().numpy().view(np.uint32)
             mont = mont[self._colmajor()]
-        return wire.put_matrix(mont, self.rows(), self.cols(), self.ndim)
+        return put[format](mont, self.rows(), self.cols(), self.ndim)
 
     @staticmethod
     def load_from(key, buf: bytes, format: str = MatrixSerializeFormat.Best) -> "CiphertextArray":
-        """DenseMatrix<Ciphertext>::LoadFrom (numpy/matrix.h:267-314) onto the
-        device of `key` (an engine Key or a HeKit)."""
-        if format != MatrixSerializeFormat.Best:
-            raise NotImplementedError("the interconnection protobuf schema (org_interconnection) is not built")
+        """DenseMatrix<Ciphertext>::LoadFrom (numpy/matrix.h:267-314), or
+        LoadFromIc (numpy/matrix.cc:78-111), onto the device of `key` (an
+        engine Key or a HeKit)."""
+        get = {MatrixSerializeFormat.Best: wire.get_matrix, MatrixSerializeFormat.Interconnection: wire.get_matrix_ic}
+        if format not in get:
+            raise ValueError(f"unknown matrix serialize format {format!r}")
         key = key.key if isinstance(key, HeKit) else key
-        rows, cols, ndim, mont = wire.get_matrix(bytes(buf), key.ct_words)
+        rows, cols, ndim, mont = get[format](bytes(buf), key.ct_words)
         shape = () if ndim == 0 else ((rows,) if ndim == 1 else (rows, cols))
         if int(np.prod(shape, dtype=np.int64)) != rows * cols:
             raise ValueError(f"a {ndim}-d matrix cannot hold {rows} x {cols} elements")

@@ -8,6 +8,9 @@ paillier_z::Ciphertext::c_ holds under yacl's 64-bit-word backends
                 (paillier_zahlen/ciphertext.h:37, phe/base/serializable_types.cc:50-70)
   matrix        msgpack [rows, cols, ndim, [item str ...]], column-major
                 (numpy/matrix.h:221-314)
+  matrix, ic    protobuf DataExchangeProtocol {scalar_type OBJECT, "ciphertext",
+                NdArray {shape, items}} (numpy/matrix.cc:39-111); the .proto
+                is not in the reference, the field numbers are wire.cpp's
   public key    msgpack [n, h_s] + index (paillier_zahlen/public_key.h:64-90)
   secret key    msgpack [lambda, mu, p, q] + index (secret_key.h:64-94)
 A BigInt inside is a msgpack str of one sign byte and the little-endian
@@ -82,6 +85,29 @@ def get_matrix(buf: bytes, words: int):
     rows, cols, ndim = matrix_shape(buf)
     out = np.zeros((rows * cols, words), np.uint32)
     check(lib().heu_wire_get_matrix(buf, len(buf), None, None, None, _ptr(out), words, rows * cols))
+    return rows, cols, ndim, out
+
+
+def put_matrix_ic(mont_colmajor: np.ndarray, rows: int, cols: int, ndim: int) -> bytes:
+    """The interconnection form of a DenseMatrix<Ciphertext> (Serialize4Ic);
+    values as for put_matrix."""
+    m = np.ascontiguousarray(mont_colmajor, dtype=np.uint32)
+    if m.ndim != 2 or m.shape[0] != rows * cols or m.shape[1] == 0:
+        raise ValueError(f"values must have shape ({rows * cols}, words)")
+    return _write(lib().heu_wire_put_matrix_ic, _ptr(m) if m.size else None, rows, cols, ndim, m.shape[1])
+
+
+def matrix_shape_ic(buf: bytes):
+    r, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    check(lib().heu_wire_get_matrix_ic(buf, len(buf), ctypes.byref(r), ctypes.byref(c), ctypes.byref(d), None, 0, 0))
+    return r.value, c.value, d.value
+
+
+def get_matrix_ic(buf: bytes, words: int):
+    """LoadFromIc: (rows, cols, ndim, values (rows * cols, words) column-major)."""
+    rows, cols, ndim = matrix_shape_ic(buf)
+    out = np.zeros((rows * cols, words), np.uint32)
+    check(lib().heu_wire_get_matrix_ic(buf, len(buf), None, None, None, _ptr(out), words, rows * cols))
     return rows, cols, ndim, out
 
 

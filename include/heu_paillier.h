@@ -242,6 +242,19 @@ int heu_wire_put_matrix(const uint32_t *mont, int64_t rows, int64_t cols, int64_
                         uint8_t *out, size_t cap, size_t *len);
 int heu_wire_get_matrix(const uint8_t *in, size_t len, int64_t *rows, int64_t *cols, int64_t *ndim, uint32_t *mont,
                         uint32_t words, size_t cap_values);
+/* DenseMatrix<Ciphertext>::Serialize4Ic / LoadFromIc (numpy/matrix.cc:39-111):
+ * the interconnection form, a protobuf DataExchangeProtocol with scalar_type
+ * OBJECT, scalar_type_name "ciphertext" and an NdArray of the shape (no entry
+ * for ndim 0, [rows] for 1, [rows, cols] for 2) and the items, each the bytes
+ * of heu_wire_put_ciphertext, column-major as above.  The reader ignores the
+ * type name and unknown fields and requires scalar_type OBJECT, an NdArray and
+ * as many items as the shape holds.  The .proto belongs to org_interconnection
+ * 0.0.1, which the reference does not vendor: the field numbers and the enum
+ * value are listed in wire.cpp. */
+int heu_wire_put_matrix_ic(const uint32_t *mont, int64_t rows, int64_t cols, int64_t ndim, uint32_t words,
+                           uint8_t *out, size_t cap, size_t *len);
+int heu_wire_get_matrix_ic(const uint8_t *in, size_t len, int64_t *rows, int64_t *cols, int64_t *ndim, uint32_t *mont,
+                           uint32_t words, size_t cap_values);
 /* phe::PublicKey / phe::SecretKey for ZPaillier.  The secret key's lambda =
  * (p-1)(q-1)/2 and mu = lambda^-1 mod n (key_generator.cc:26-60) are computed
  * from p and q when written and checked against them when read.  Readers

@@ -2,7 +2,8 @@
 on the device, against closed forms.
 
   serialize / load_from   numpy/matrix.h:221-314 (column-major items, each the
-                          variant bytes of a Montgomery-form ciphertext), and
+                          variant bytes of a Montgomery-form ciphertext), the
+                          interconnection form (numpy/matrix.cc:39-111), and
                           the key round trip of pylib/phe_test.py:279-294 and
                           numpy_test.py:299-321 (serialize on one side, set up
                           from the bytes, decrypt on the other);
@@ -65,6 +66,21 @@ def test_matrix_round_trip(hnp, kit, shape):
     assert back.shape == tuple(shape)
     assert torch.equal(back.data, ct.data)
     assert np.array_equal(kit.decryptor().decrypt(back), m)
+
+
+def test_serialize_ic(hnp, kit):
+    # numpy_test.py:323-337: the interconnection bytes differ from the default
+    # form, and each format loads back to the same values
+    fmt = hnp.MatrixSerializeFormat
+    for m in (np.array([[16, 19], [36, 43]], dtype=np.int64), np.array([100], dtype=np.int64),
+              np.array(-5, dtype=np.int64)):
+        ct = kit.encryptor().encrypt(m, seed=12)
+        buf1, buf2 = ct.serialize(fmt.Interconnection), ct.serialize()
+        assert buf1 != buf2
+        back = hnp.CiphertextArray.load_from(kit, buf1, fmt.Interconnection)
+        assert back.shape == m.shape and torch.equal(back.data, ct.data)
+        assert np.array_equal(kit.decryptor().decrypt(back), m)
+        assert torch.equal(hnp.CiphertextArray.load_from(kit, buf2).data, ct.data)
 
 
 def test_keys_and_ciphertexts_cross_parties(hnp, kit):

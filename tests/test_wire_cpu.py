@@ -121,3 +121,107 @@ def test_keys(wire):
         wire.get_secret_key(tampered)
     with pytest.raises(HeuError):
         wire.get_public_key(pk[:-8] + (1).to_bytes(8, "little"))
+
+
+# ---- interconnection form (numpy/matrix.cc:39-111) ---------------------------
+# The org_interconnection .proto is not in the reference tree; the messages
+# below restate the field numbers the engine states (wire.cpp), and protobuf's
+# own implementation encodes and decodes against them, so the encoding itself
+# (tags, varints, packing, nesting, field order) is checked independently.
+# Parity unpinned: the field numbers and SCALAR_TYPE_OBJECT's value.
+IC_OBJECT = 12
+
+
+@pytest.fixture(scope="module")
+def ic_pb():
+    pytest.importorskip("google.protobuf")
+    from google.protobuf import descriptor_pb2, descriptor_pool, message_factory
+
+    fd = descriptor_pb2.FileDescriptorProto(name="ic_test.proto", package="ic_test", syntax="proto3")
+    nd = fd.message_type.add(name="NdArray")
+    F = descriptor_pb2.FieldDescriptorProto
+    nd.field.add(name="shape", number=1, type=F.TYPE_INT64, label=F.LABEL_REPEATED)
+    nd.field.add(name="items", number=3, type=F.TYPE_BYTES, label=F.LABEL_REPEATED)
+    nd.field.add(name="item_buffer", number=2, type=F.TYPE_BYTES, label=F.LABEL_OPTIONAL)  # unknown to the engine
+    dep = fd.message_type.add(name="DataExchangeProtocol")
+    dep.field.add(name="scalar_type", number=1, type=F.TYPE_INT32, label=F.LABEL_OPTIONAL)
+    dep.field.add(name="scalar_type_name", number=2, type=F.TYPE_STRING, label=F.LABEL_OPTIONAL)
+    dep.oneof_decl.add(name="container")
+    dep.field.add(name="v_other", number=3, type=F.TYPE_BYTES, label=F.LABEL_OPTIONAL, oneof_index=0)
+    dep.field.add(name="v_ndarray", number=4, type=F.TYPE_MESSAGE, type_name=".ic_test.NdArray",
+                  label=F.LABEL_OPTIONAL, oneof_index=0)
+    pool = descriptor_pool.DescriptorPool()
+    pool.Add(fd)
+    get = message_factory.GetMessageClass
+    return get(pool.FindMessageTypeByName("ic_test.DataExchangeProtocol"))
+
+
+def ic_message(ic_pb, rows, cols, ndim, items, scalar_type=IC_OBJECT):
+    m = ic_pb(scalar_type=scalar_type, scalar_type_name="ciphertext")
+    m.v_ndarray.SetInParent()
+    m.v_ndarray.shape.extend([rows, cols][:ndim])
+    m.v_ndarray.items.extend(items)
+    return m
+
+
+@pytest.mark.parametrize("rows,cols,ndim", [(2, 3, 2), (3, 1, 1), (1, 1, 0), (0, 1, 1), (200, 1, 1), (0, 4, 2)])
+def test_matrix_ic_layout(wire, ic_pb, rows, cols, ndim):
+    rng = np.random.default_rng(rows * 11 + cols + ndim)
+    vals = [int.from_bytes(rng.bytes(512), "little") >> int(rng.integers(0, 4000)) for _ in range(rows * cols)]
+    mont = np.stack([limbs(v, 128) for v in vals]) if vals else np.zeros((0, 128), np.uint32)
+    b = wire.put_matrix_ic(mont, rows, cols, ndim)
+    items = [wire.put_ciphertext(limbs(v, 128)) for v in vals]
+    # protobuf writes the same bytes for the same message, and reads ours back
+    want = ic_message(ic_pb, rows, cols, ndim, items)
+    assert want.SerializeToString() == b
+    got = ic_pb.FromString(b)
+    assert got.scalar_type == IC_OBJECT and got.scalar_type_name == "ciphertext"
+    assert got.WhichOneof("container") == "v_ndarray"
+    assert list(got.v_ndarray.shape) == [rows, cols][:ndim] and list(got.v_ndarray.items) == items
+    r, c, d, back = wire.get_matrix_ic(b, 128)
+    assert (r, c, d) == (rows, cols, ndim) and np.array_equal(back, mont)
+    # the msgpack form differs (numpy_test.py:323-327)
+    assert wire.put_matrix(mont, rows, cols, ndim) != b
+
+
+def test_matrix_ic_reader_is_lenient_where_protobuf_is(wire, ic_pb):
+    vals = [5, 7, 11, 13, 17, 19]
+    items = [wire.put_ciphertext(limbs(v, 128)) for v in vals]
+    mont = np.stack([limbs(v, 128) for v in vals])
+    m = ic_message(ic_pb, 2, 3, 2, items)
+    m.v_ndarray.item_buffer = b"ignored"  # a field the engine does not know
+    assert np.array_equal(wire.get_matrix_ic(m.SerializeToString(), 128)[3], mont)
+
+    # by hand: fields in reverse order, shape unpacked, a two-byte varint length
+    def tag(f, w):
+        return bytes([(f << 3) | w])
+
+    nd = b"".join(tag(3, 2) + bytes([len(it)]) + it for it in items) + tag(1, 0) + b"\x02" + tag(1, 0) + b"\x03"
+    assert len(nd) < 128
+    msg = tag(4, 2) + bytes([len(nd) | 0x80, 0]) + nd + tag(2, 2) + b"\x0aciphertext" + tag(1, 0) + bytes([IC_OBJECT])
+    assert ic_pb.FromString(msg).v_ndarray.shape == [2, 3]  # protobuf reads it too
+    assert wire.matrix_shape_ic(msg) == (2, 3, 2)
+    assert np.array_equal(wire.get_matrix_ic(msg, 128)[3], mont)
+
+
+def test_matrix_ic_rejects(wire, ic_pb):
+    from heu_amd._native import HeuError
+
+    items = [wire.put_ciphertext(limbs(v, 128)) for v in (1, 2, 3)]
+    bad = [
+        ic_message(ic_pb, 3, 1, 1, items, scalar_type=1).SerializeToString(),  # not OBJECT
+        ic_message(ic_pb, 2, 1, 1, items).SerializeToString(),  # shape and len do not match
+        ic_message(ic_pb, 3, 1, 1, items[:2] + [b"\x91\xa1\x00"]).SerializeToString(),  # item not a ciphertext
+        ic_pb(scalar_type=IC_OBJECT, v_other=b"x").SerializeToString(),  # another container
+        ic_pb(scalar_type=IC_OBJECT).SerializeToString(),  # no container
+        ic_message(ic_pb, 3, 1, 1, items).SerializeToString()[:-3],  # truncated
+        b"\xff\xff\xff",
+    ]
+    three_d = ic_message(ic_pb, 1, 1, 2, items[:1])
+    three_d.v_ndarray.shape.append(1)
+    bad.append(three_d.SerializeToString())
+    for b in bad:
+        with pytest.raises(HeuError):
+            wire.get_matrix_ic(b, 128)
+    with pytest.raises(HeuError):  # a 0-d matrix is 1 x 1
+        wire.put_matrix_ic(np.zeros((2, 128), np.uint32), 2, 1, 0)
