@@ -336,18 +336,19 @@ def main():
     # engine form: the same ct+ct and axis-0 sum on values kept on the device as
     # Montgomery limbs between calls (one product per element or term)
     ct_m = key.to_mform(ct_res)
-    key.add_mform(ct_m, ct_m)
+    ct_m2 = key.to_mform(torch.flip(ct_res, [0]))  # a second operand array, so ct+ct reads two
+    key.add_mform(ct_m, ct_m2)
     key.sum_mform(ct_m, offs)
     ctx.sync()
     ctx.profile(True)
     for _ in range(2):
-        key.add_mform(ct_m, ct_m)
+        key.add_mform(ct_m, ct_m2)
         key.sum_mform(ct_m, offs)
     ctx.sync()
     mm_ms, mm_launches = ctx.profile_read("mulmod_m")
     summ_ms, _ = ctx.profile_read("reduce_sum")
     ctx.profile(False)
-    del ct_m
+    del ct_m, ct_m2
     extra_rows = [("reduce_sum", (sum_ms, 0)), ("mulmod_m", (mm_ms, mm_launches)), ("reduce_sum_m", (summ_ms, 0))]
     kernels = {}
     traffic_all = {}
