@@ -8,13 +8,21 @@
 // through a combining queue: a thread that finds no other thread running the
 // queue takes every queued call, and runs each group of calls with the same
 // operation, key and parameters as one staged engine call.
+//
+// Host side of a call: operands are packed into, and results read from,
+// pooled page-locked buffers (no fresh allocation or page faults per call,
+// DMA-speed copies), and the per-element conversions (validation, limb
+// copies, BigInt construction, randomness) are split over the usable cores.
 #include "paillier_hip.h"
 
 #include <hip/hip_runtime_api.h>
 #include <openssl/bn.h>
 #include <openssl/crypto.h>
 #include <openssl/rand.h>
+#include <sched.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,6 +31,7 @@
 #include <exception>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <stdexcept>
 
@@ -275,7 +284,14 @@ class DeviceCall {
     return reinterpret_cast<T*>(base_ + offs_[h]);
   }
 
+  // An exception between upload() and finish() leaves copies in flight that
+  // read or write the callers' host buffers: wait for them before those go.
+  ~DeviceCall() {
+    if (started_ && !finished_) (void)hipStreamSynchronize(rt_.stream);
+  }
+
   void upload() {
+    started_ = true;
     base_ = rt_.reserve(total_ ? total_ : 256);
     for (auto& u : ups_)
       if (u.bytes)
@@ -293,7 +309,10 @@ class DeviceCall {
              "hipMemcpyAsync D2H");
   }
 
-  void finish() { ck(heu_ctx_sync(rt_.ctx)); }
+  void finish() {
+    finished_ = true;
+    ck(heu_ctx_sync(rt_.ctx));
+  }
 
  private:
   struct Up {
@@ -319,6 +338,138 @@ class DeviceCall {
   std::vector<Piece> pieces_;
   size_t total_ = 0;
   char* base_ = nullptr;
+  bool started_ = false, finished_ = false;
+};
+
+// ---- host staging and threads
+
+// Threads for one call's per-element host work: the usable cores (the
+// affinity mask), at most 16.
+size_t host_threads() {
+  static const size_t n = [] {
+    cpu_set_t set;
+    size_t c = 1;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) c = (size_t)CPU_COUNT(&set);
+    return std::max<size_t>(1, std::min<size_t>(c, 16));
+  }();
+  return n;
+}
+
+// Elements per thread below which a loop stays on the calling thread.
+constexpr size_t kGrain = 2048;
+
+// f(begin, end) over [0, n), split into contiguous chunks on up to
+// host_threads() threads, the caller's among them.  An exception ends its
+// chunk; once every chunk ended, the exception of the lowest chunk is
+// rethrown, which is the first failing element in order, as a serial loop
+// would report it.
+template <class F>
+void par_for(size_t n, size_t grain, const F& f) {
+  const size_t t = std::min(host_threads(), n / std::max<size_t>(grain, 1));
+  if (t <= 1) return f(size_t(0), n);
+  std::vector<std::exception_ptr> err(t);
+  auto chunk = [&](size_t i) {
+    try {
+      f(n * i / t, n * (i + 1) / t);
+    } catch (...) {
+      err[i] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  th.reserve(t - 1);
+  size_t next = 1;
+  try {
+    for (; next < t; ++next) th.emplace_back(chunk, next);
+  } catch (...) {
+    // no more threads: the chunks left run here
+  }
+  chunk(0);
+  for (size_t i = next; i < t; ++i) chunk(i);
+  for (auto& x : th) x.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
+// Page-locked host buffers, pooled for reuse.  A call takes one per operand
+// or result array and gives it back when it returns.  Capacities are powers
+// of two; a request reuses an idle buffer of at most 4x its size.  At most
+// kKeep bytes stay idle, the rest is freed.  Never destroyed, like Runtime.
+struct HostPool {
+  static constexpr size_t kKeep = size_t(2) << 30;
+  std::mutex mu;
+  std::multimap<size_t, void*> idle;
+  size_t idle_bytes = 0;
+
+  static HostPool& get() {
+    static HostPool* p = new HostPool;
+    return *p;
+  }
+
+  void* take(size_t bytes, size_t* cap) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = idle.lower_bound(bytes);
+      if (it != idle.end() && it->first / 4 <= bytes) {
+        void* p = it->second;
+        *cap = it->first;
+        idle_bytes -= it->first;
+        idle.erase(it);
+        return p;
+      }
+    }
+    size_t c = size_t(1) << 16;
+    while (c < bytes) c <<= 1;
+    Runtime::get();  // the device the context uses is current
+    void* p = nullptr;
+    hip_ck(hipHostMalloc(&p, c, hipHostMallocDefault), "hipHostMalloc");
+    *cap = c;
+    return p;
+  }
+
+  void give(void* p, size_t cap) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (idle_bytes + cap <= kKeep) {
+        idle.emplace(cap, p);
+        idle_bytes += cap;
+        return;
+      }
+    }
+    (void)hipHostFree(p);
+  }
+};
+
+// n values of a trivially copyable T in a pooled page-locked buffer
+// (contents undefined until written).
+template <class T>
+class Staged {
+ public:
+  Staged() = default;
+  explicit Staged(size_t n) : n_(n) {
+    if (n) p_ = static_cast<T*>(HostPool::get().take(n * sizeof(T), &cap_));
+  }
+  Staged(Staged&& o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  Staged& operator=(Staged&& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+    std::swap(cap_, o.cap_);
+    return *this;
+  }
+  Staged(const Staged&) = delete;
+  ~Staged() {
+    if (p_) HostPool::get().give(p_, cap_);
+  }
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  size_t size() const { return n_; }
+  T& operator[](size_t i) { return p_[i]; }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0, cap_ = 0;
 };
 
 // Operations the combining queue runs (elementwise, fixed bytes per element).
@@ -483,14 +634,18 @@ std::vector<uint32_t> words_of(const BigInt& v, size_t words) {
   return w;
 }
 
-// The reference draws r with key_size/2 bits (encryptor.cc:26-27).
-std::vector<uint32_t> random_r(const DeviceKey& dk, size_t count) {
-  std::vector<uint32_t> r(count * dk.r_words);
-  if (!r.empty() && RAND_bytes(reinterpret_cast<unsigned char*>(r.data()), (int)(r.size() * 4)) != 1)
-    throw_msg("RAND_bytes failed");
+// The reference draws r with key_size/2 bits (encryptor.cc:26-27).  OpenSSL's
+// RAND_bytes is thread-safe (a DRBG per thread), so chunks draw in parallel.
+Staged<uint32_t> random_r(const DeviceKey& dk, size_t count) {
+  const size_t rw = dk.r_words;
+  Staged<uint32_t> r(count * rw);
   const unsigned top = (dk.key_bits / 2) % 32;
-  if (top)
-    for (size_t i = 0; i < count; ++i) r[i * dk.r_words + dk.r_words - 1] &= (1u << top) - 1u;
+  par_for(count, kGrain, [&](size_t b, size_t e) {
+    if (e > b && RAND_bytes(reinterpret_cast<unsigned char*>(r.data() + b * rw), (int)((e - b) * rw * 4)) != 1)
+      throw_msg("RAND_bytes failed");
+    if (top)
+      for (size_t i = b; i < e; ++i) r[i * rw + rw - 1] &= (1u << top) - 1u;
+  });
   return r;
 }
 
@@ -525,54 +680,74 @@ const DeviceKey& dev(const PublicKey& pk) {
 }
 
 // Evaluator VALIDATE (evaluator.cc:20-22): 0 <= c < n^2, packed into rows.
-std::vector<uint32_t> pack(const DeviceKey& dk, ConstSpan<Ciphertext> cts) {
+Staged<uint32_t> pack(const DeviceKey& dk, ConstSpan<Ciphertext> cts) {
   const size_t W = dk.ct_words;
-  std::vector<uint32_t> flat(cts.size() * W);
-  for (size_t i = 0; i < cts.size(); ++i) {
-    const auto& c = cts[i]->c_;
-    if (c.size() != W) throw_msg("Evaluator: Invalid ciphertext (wrong limb count)");
-    int cmp = 0;
-    for (size_t j = W; j-- > 0 && cmp == 0;) cmp = c[j] < dk.n2[j] ? -1 : (c[j] > dk.n2[j] ? 1 : 0);
-    if (cmp >= 0) throw_msg("Evaluator: Invalid ciphertext");
-    std::memcpy(flat.data() + i * W, c.data(), W * 4);
-  }
+  Staged<uint32_t> flat(cts.size() * W);
+  par_for(cts.size(), kGrain, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      const auto& c = cts[i]->c_;
+      if (c.size() != W) throw_msg("Evaluator: Invalid ciphertext (wrong limb count)");
+      int cmp = 0;
+      for (size_t j = W; j-- > 0 && cmp == 0;) cmp = c[j] < dk.n2[j] ? -1 : (c[j] > dk.n2[j] ? 1 : 0);
+      if (cmp >= 0) throw_msg("Evaluator: Invalid ciphertext");
+      std::memcpy(flat.data() + i * W, c.data(), W * 4);
+    }
+  });
   return flat;
 }
 
-std::vector<uint32_t> pack(const DeviceKey& dk, Span<Ciphertext> cts) {
+Staged<uint32_t> pack(const DeviceKey& dk, Span<Ciphertext> cts) {
   return pack(dk, ConstSpan<Ciphertext>(cts.data(), cts.size()));
 }
 
-std::vector<Ciphertext> unpack(const DeviceKey& dk, const std::vector<uint32_t>& flat, size_t count) {
+std::vector<Ciphertext> unpack(const DeviceKey& dk, const uint32_t* flat, size_t count) {
   std::vector<Ciphertext> out(count);
   const size_t W = dk.ct_words;
-  for (size_t i = 0; i < count; ++i) out[i].c_.assign(flat.begin() + i * W, flat.begin() + (i + 1) * W);
+  par_for(count, kGrain, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) out[i].c_.assign(flat + i * W, flat + (i + 1) * W);
+  });
   return out;
+}
+
+std::vector<Ciphertext> unpack(const DeviceKey& dk, const Staged<uint32_t>& flat, size_t count) {
+  return unpack(dk, flat.data(), count);
 }
 
 // Plaintexts as int64 when all fit, else two's complement rows of n_words.
 struct PtBatch {
   bool small = true;
-  std::vector<int64_t> i64;
+  Staged<int64_t> i64;
   std::vector<uint32_t> rows;
   uint32_t words = 0;
 };
 
 PtBatch pack_pts(const DeviceKey& dk, ConstSpan<Plaintext> pts, const BigInt* bound, bool negate) {
   PtBatch b;
-  for (const Plaintext* p : pts) {
-    if (bound && p->CompareAbs(*bound) > 0)
-      throw_msg("plaintext out of range, message=" + p->ToHexString() + ", max (abs)=" + bound->ToHexString());
-    b.small = b.small && p->FitsInt64() && !(negate && p->ToInt64() == std::numeric_limits<int64_t>::min());
-  }
+  const size_t n = pts.size();
+  std::atomic<bool> small{true};
+  par_for(n, kGrain, [&](size_t lo, size_t hi) {
+    bool s = true;
+    for (size_t i = lo; i < hi; ++i) {
+      const Plaintext* p = pts[i];
+      if (bound && p->CompareAbs(*bound) > 0)
+        throw_msg("plaintext out of range, message=" + p->ToHexString() + ", max (abs)=" + bound->ToHexString());
+      s = s && p->FitsInt64() && !(negate && p->ToInt64() == std::numeric_limits<int64_t>::min());
+    }
+    if (!s) small = false;
+  });
+  b.small = small;
   if (b.small) {
-    b.i64.reserve(pts.size());
-    for (const Plaintext* p : pts) b.i64.push_back(negate ? -p->ToInt64() : p->ToInt64());
+    b.i64 = Staged<int64_t>(n);
+    par_for(n, kGrain, [&](size_t lo, size_t hi) {
+      for (size_t i = lo; i < hi; ++i) b.i64[i] = negate ? -pts[i]->ToInt64() : pts[i]->ToInt64();
+    });
     return b;
   }
   b.words = dk.n_words;
-  b.rows.resize(pts.size() * b.words);
-  for (size_t i = 0; i < pts.size(); ++i) (negate ? -*pts[i] : *pts[i]).ToWords(b.rows.data() + i * b.words, b.words);
+  b.rows.resize(n * b.words);
+  par_for(n, kGrain, [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) (negate ? -*pts[i] : *pts[i]).ToWords(b.rows.data() + i * b.words, b.words);
+  });
   return b;
 }
 
@@ -601,7 +776,7 @@ std::vector<Ciphertext> add_ct_pt(const DeviceKey& dk, const BigInt& bound, Cons
   if (cnt == 0) return {};
   auto ca = pack(dk, a);
   PtBatch pb = pack_pts(dk, b, &bound, negate_b);
-  std::vector<uint32_t> res(ca.size());
+  Staged<uint32_t> res(ca.size());
   if (pb.small) {
     Job j = make_job(kJobAddI64, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
     add_in(j, ca.data(), dk.ct_words * 4);
@@ -628,7 +803,7 @@ std::vector<Ciphertext> ct_unary(const DeviceKey& dk, ConstSpan<Ciphertext> a, i
   const size_t cnt = a.size();
   if (cnt == 0) return {};
   auto ca = pack(dk, a);
-  std::vector<uint32_t> res(ca.size());
+  Staged<uint32_t> res(ca.size());
   Job j = make_job(op, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
   add_in(j, ca.data(), dk.ct_words * 4);
   submit(j);
@@ -640,7 +815,7 @@ std::vector<Ciphertext> ct_binary(const DeviceKey& dk, ConstSpan<Ciphertext> a, 
   const size_t cnt = a.size();
   if (cnt == 0) return {};
   auto ca = pack(dk, a), cb = pack(dk, b);
-  std::vector<uint32_t> res(ca.size());
+  Staged<uint32_t> res(ca.size());
   Job j = make_job(op, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
   add_in(j, ca.data(), dk.ct_words * 4);
   add_in(j, cb.data(), dk.ct_words * 4);
@@ -833,7 +1008,7 @@ std::vector<Ciphertext> Encryptor::EncryptZero(int64_t size) const {
   const size_t cnt = (size_t)size;
   if (cnt == 0) return {};
   auto r = random_r(dk, cnt);
-  std::vector<uint32_t> res(cnt * dk.ct_words);
+  Staged<uint32_t> res(cnt * dk.ct_words);
   Job j = make_job(kJobEncZero, dk.key, dk.r_words, cnt, res.data(), dk.ct_words * 4);
   add_in(j, r.data(), dk.r_words * 4);
   submit(j);
@@ -846,7 +1021,7 @@ std::vector<Ciphertext> Encryptor::EncryptWithR(ConstSpan<Plaintext> pts, const 
   if (cnt == 0) return {};
   if (r_words == 0 || !r) throw_msg("EncryptWithR: empty randomness");
   PtBatch pb = pack_pts(dk, pts, &pk_.n_half_, false);
-  std::vector<uint32_t> res(cnt * dk.ct_words);
+  Staged<uint32_t> res(cnt * dk.ct_words);
   if (pb.small) {
     Job j = make_job(kJobEncI64, dk.key, (uint32_t)r_words, cnt, res.data(), dk.ct_words * 4);
     add_in(j, pb.i64.data(), 8);
@@ -869,10 +1044,12 @@ std::vector<Ciphertext> Encryptor::EncryptWithR(ConstSpan<Plaintext> pts, const 
 std::vector<Ciphertext> Encryptor::Encrypt(ConstSpan<Plaintext> pts) const {
   const DeviceKey& dk = dev(pk_);
   // range check first (encryptor.cc:38-40), so no randomness is drawn for a bad batch
-  for (const Plaintext* p : pts)
-    if (p->CompareAbs(pk_.n_half_) > 0)
-      throw_msg("message number out of range, message=" + p->ToHexString() + ", max (abs)=" +
-                pk_.n_half_.ToHexString());
+  par_for(pts.size(), kGrain, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i)
+      if (pts[i]->CompareAbs(pk_.n_half_) > 0)
+        throw_msg("message number out of range, message=" + pts[i]->ToHexString() + ", max (abs)=" +
+                  pk_.n_half_.ToHexString());
+  });
   auto r = random_r(dk, pts.size());
   return EncryptWithR(pts, r.data(), dk.r_words);
 }
@@ -887,14 +1064,14 @@ std::pair<std::vector<Ciphertext>, std::vector<std::string>> Encryptor::EncryptW
   auto cts = EncryptWithR(pts, r.data(), dk.r_words);
   std::vector<std::string> audit(cnt);
   if (cnt == 0) return {cts, audit};
-  std::vector<uint32_t> rn(cnt * dk.ct_words);
+  Staged<uint32_t> rn(cnt * dk.ct_words);
   {
     Job j = make_job(kJobEncZero, dk.key, dk.r_words, cnt, rn.data(), dk.ct_words * 4);
     add_in(j, r.data(), dk.r_words * 4);
     submit(j);
   }
   for (size_t i = 0; i < cnt; ++i) {
-    std::vector<uint32_t> one(rn.begin() + i * dk.ct_words, rn.begin() + (i + 1) * dk.ct_words);
+    std::vector<uint32_t> one(rn.data() + i * dk.ct_words, rn.data() + (i + 1) * dk.ct_words);
     audit[i] = "p:" + pts[i]->ToHexString() + ",rn:" + hex_words(mont_words(one, pk_.n_square_, true)) +
                ",c:" + hex_words(mont_words(cts[i].c_, pk_.n_square_, true));
   }
@@ -927,11 +1104,13 @@ void Decryptor::Decrypt(ConstSpan<Ciphertext> in_cts, Span<Plaintext> out_pts) c
   if (cnt == 0) return;
   auto ca = pack(dk, in_cts);
   const uint32_t ow = dk.n_words;  // |m| <= n/2 always fits n_words in two's complement
-  std::vector<uint32_t> res(cnt * ow);
+  Staged<uint32_t> res(cnt * ow);
   Job j = make_job(kJobDecrypt, dk.key, ow, cnt, res.data(), ow * 4);
   add_in(j, ca.data(), dk.ct_words * 4);
   submit(j);
-  for (size_t i = 0; i < cnt; ++i) *out_pts[i] = BigInt::FromWords(res.data() + i * ow, ow, true);
+  par_for(cnt, kGrain, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) *out_pts[i] = BigInt::FromWords(res.data() + i * ow, ow, true);
+  });
 }
 
 std::vector<Plaintext> Decryptor::Decrypt(ConstSpan<Ciphertext> cts) const {
@@ -960,7 +1139,7 @@ void Evaluator::Randomize(Span<Ciphertext> ct) const {
   if (cnt == 0) return;
   auto ca = pack(dk, ct);
   auto r = random_r(dk, cnt);
-  std::vector<uint32_t> res(ca.size());
+  Staged<uint32_t> res(ca.size());
   Job j = make_job(kJobRandomize, dk.key, dk.r_words, cnt, res.data(), dk.ct_words * 4);
   add_in(j, ca.data(), dk.ct_words * 4);
   add_in(j, r.data(), dk.r_words * 4);
@@ -1041,18 +1220,26 @@ std::vector<Ciphertext> Evaluator::Mul(ConstSpan<Ciphertext> a, ConstSpan<Plaint
   check_same(a.size(), b.size());
   const size_t cnt = a.size();
   if (cnt == 0) return {};
-  bool small = true;
-  size_t bits = 1;
-  for (const Plaintext* p : b) {
-    small = small && p->FitsInt64();
-    bits = std::max(bits, p->BitCount());
-  }
+  std::atomic<bool> small{true};
+  std::atomic<size_t> bits{1};
+  par_for(cnt, kGrain, [&](size_t lo, size_t hi) {
+    bool s = true;
+    size_t nb = 1;
+    for (size_t i = lo; i < hi; ++i) {
+      s = s && b[i]->FitsInt64();
+      nb = std::max(nb, b[i]->BitCount());
+    }
+    if (!s) small = false;
+    for (size_t cur = bits.load(); nb > cur && !bits.compare_exchange_weak(cur, nb);) {
+    }
+  });
   auto ca = pack(dk, a);
-  std::vector<uint32_t> res(ca.size());
+  Staged<uint32_t> res(ca.size());
   if (small) {
-    std::vector<int64_t> k64;
-    k64.reserve(cnt);
-    for (const Plaintext* p : b) k64.push_back(p->ToInt64());
+    Staged<int64_t> k64(cnt);
+    par_for(cnt, kGrain, [&](size_t lo, size_t hi) {
+      for (size_t i = lo; i < hi; ++i) k64[i] = b[i]->ToInt64();
+    });
     Job j = make_job(kJobMulI64, dk.key, 0, cnt, res.data(), dk.ct_words * 4);
     add_in(j, ca.data(), dk.ct_words * 4);
     add_in(j, k64.data(), 8);
