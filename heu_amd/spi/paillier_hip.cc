@@ -358,6 +358,11 @@ size_t host_threads() {
 // Elements per thread below which a loop stays on the calling thread.
 constexpr size_t kGrain = 2048;
 
+// Helper threads running now, across all calls: concurrent calls (HEU's
+// worker threads each converting a chunk) share host_threads() - 1 helpers
+// instead of each starting its own.
+std::atomic<size_t> g_helpers{0};
+
 // f(begin, end) over [0, n), split into contiguous chunks on up to
 // host_threads() threads, the caller's among them.  An exception ends its
 // chunk; once every chunk ended, the exception of the lowest chunk is
@@ -365,8 +370,21 @@ constexpr size_t kGrain = 2048;
 // would report it.
 template <class F>
 void par_for(size_t n, size_t grain, const F& f) {
-  const size_t t = std::min(host_threads(), n / std::max<size_t>(grain, 1));
+  const size_t want = std::min(host_threads(), n / std::max<size_t>(grain, 1));
+  size_t got = 0;
+  if (want > 1) {
+    const size_t budget = host_threads() - 1;
+    size_t cur = g_helpers.load();
+    do {
+      got = std::min(want - 1, cur < budget ? budget - cur : 0);
+    } while (!g_helpers.compare_exchange_weak(cur, cur + got));
+  }
+  const size_t t = got + 1;
   if (t <= 1) return f(size_t(0), n);
+  struct Release {
+    size_t k;
+    ~Release() { g_helpers -= k; }
+  } release{got};
   std::vector<std::exception_ptr> err(t);
   auto chunk = [&](size_t i) {
     try {

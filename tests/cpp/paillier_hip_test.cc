@@ -388,6 +388,70 @@ void VectorizedClosedForms(const Keys& k, size_t count) {
   EXPECT_THROW(evaluator.Add(cs(pc), cs(shorter)));
 }
 
+// Spans long enough that the SPI splits its host work (packing, checks,
+// conversions, result objects) over threads: every path still gives the
+// closed forms, checked through decryption, and an error names the first bad
+// element in span order, as a serial loop would.
+void SpansSplitOverThreads(const Keys& k) {
+  const size_t count = 20000;
+  Encryptor encryptor(k.pk);
+  Evaluator evaluator(k.pk);
+  Decryptor decryptor(k.pk, k.sk);
+  const BigInt& n = k.pk.n_;
+  const int nbits = (int)n.BitCount();
+  std::mt19937_64 g(7);
+  std::vector<BigInt> big, small, k64, kbig;
+  for (size_t i = 0; i < count; ++i) {
+    big.push_back(random_bits(g, (int)(g() % (uint64_t)(nbits - 2)) + 1, true));
+    small.push_back(BigInt((int64_t)g()));
+    k64.push_back(BigInt((int64_t)(int32_t)g()));
+    kbig.push_back(random_bits(g, 100, true));
+  }
+  auto wrap = [&](const BigInt& v) {
+    BigInt m = nnmod(v, n);
+    return m > k.pk.n_half_ ? m - n : m;
+  };
+  auto pb = cptrs(big), ps = cptrs(small), pk64 = cptrs(k64), pkb = cptrs(kbig);
+  auto cb = encryptor.Encrypt(cs(pb));  // plaintexts beyond int64
+  auto csm = encryptor.Encrypt(cs(ps));
+  auto pcb = cptrs(cb), pcs = cptrs(csm);
+  auto db = decryptor.Decrypt(cs(pcb)), dsm = decryptor.Decrypt(cs(pcs));
+  auto dm64 = decryptor.Decrypt(cs(cptrs(evaluator.Mul(cs(pcs), cs(pk64)))));
+  auto dmbig = decryptor.Decrypt(cs(cptrs(evaluator.Mul(cs(pcs), cs(pkb)))));
+  auto dadd = decryptor.Decrypt(cs(cptrs(evaluator.Add(cs(pcb), cs(ps)))));
+  auto dsub = decryptor.Decrypt(cs(cptrs(evaluator.Sub(cs(pcb), cs(pcs)))));
+  bool ok = db.size() == count && dsm.size() == count;
+  for (size_t i = 0; ok && i < count; ++i)
+    ok = db[i] == big[i] && dsm[i] == small[i] && dm64[i] == wrap(small[i] * k64[i]) &&
+         dmbig[i] == wrap(small[i] * kbig[i]) && dadd[i] == wrap(big[i] + small[i]) &&
+         dsub[i] == wrap(big[i] - small[i]);
+  EXPECT_TRUE(ok);
+
+  // out-of-range plaintexts at 4100 (past the first chunk) and 19000 (in the
+  // last): the message names the one at 4100
+  auto names_first = [&](const std::function<void()>& call, const BigInt& first) {
+    try {
+      call();
+    } catch (const std::exception& e) {
+      return std::string(e.what()).find("message=" + first.ToHexString() + ",") != std::string::npos;
+    }
+    return false;
+  };
+  std::vector<BigInt> bad = small;
+  bad[4100] = k.pk.n_half_ + BigInt(7);
+  bad[19000] = k.pk.n_half_ + BigInt(9);
+  auto pbad = cptrs(bad);
+  EXPECT_TRUE(names_first([&] { encryptor.Encrypt(cs(pbad)); }, bad[4100]));
+  EXPECT_TRUE(names_first([&] { evaluator.Add(cs(pcs), cs(pbad)); }, bad[4100]));
+  // a ciphertext >= n^2 deep in the span is rejected
+  auto cbad = csm;
+  std::vector<uint32_t> n2w(cbad[0].c_.size() + 1);
+  k.pk.n_square_.ToWords(n2w.data(), n2w.size());
+  cbad[15000].c_.assign(n2w.begin(), n2w.end() - 1);
+  EXPECT_THROW(evaluator.Add(cs(cptrs(cbad)), cs(pcs)));
+  EXPECT_THROW(decryptor.Decrypt(cs(cptrs(cbad))));
+}
+
 // HEU's numpy layer calls the span methods from its worker threads, one chunk
 // per thread (numpy/encryptor.cc:28, numpy/evaluator.cc:124).  Calls lined up
 // while the queue is held run as one engine call per (operation, key), and
@@ -589,6 +653,7 @@ int main() {
   run("KeyGenerator.Generate(3072)", [&] { KeyGenerator::Generate(3072, &k3072.sk, &k3072.pk); });
   run("ZPaillierTest.MinMaxDecrypt(3072)", [&] { MinMaxDecrypt(k3072); });
   run("Vectorized.ClosedForms(3072, 65)", [&] { VectorizedClosedForms(k3072, 65); });
+  run("Vectorized.SpansSplitOverThreads(2048, 20000)", [&] { SpansSplitOverThreads(seeded); });
   run("Concurrent chunks coalesce (2048 + 3072 keys)", [&] { ConcurrentChunks(seeded, k3072); });
   run("Vectorized.SpanRates(2048, 65536)", [&] { SpanRates(seeded, 65536); });
   run("Decryptor rejects an unpaired key", [&] { EXPECT_THROW(Decryptor(k2048.pk, k3072.sk)); });
