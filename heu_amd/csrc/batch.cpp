@@ -221,21 +221,19 @@ void plan_segmented(Builder& b, heu_key* key, const heu_task& t, WorkList& wl) {
         o->reduce_mont(st, mod, pneg, ch, nch, cneg);
       }, 2);
     } else {
-      b.then([=](hipStream_t st) { o->reduce_sum(st, mod, false, ppos, ch, nch, cpos, nullptr, nullptr); });
+      b.then([=](hipStream_t st) { o->reduce_sum(st, mod, false, ppos, ch, nch, cpos, nullptr); });
     }
   }
   // the last level holds one partial per segment, in segment order
   const Level& top = lv.back();
   uint32_t* out = static_cast<uint32_t*>(t.out);
   if (!dot) {
-    std::vector<int64_t> nt(len.begin(), len.end());
-    const int64_t* d_nt = b.meta(nt);
     uint32_t* pos = top.pos;
-    b.then([=](hipStream_t st) { o->seg_finish(st, mod, pos, d_nt, (long)segs, 0, out); });
+    b.then([=](hipStream_t st) { o->seg_finish(st, mod, pos, (long)segs, out); });
   } else {
     uint32_t* nz = b.alloc<uint32_t>(segs * (size_t)key->ct_words * 4);
     uint32_t *pos = top.pos, *neg = top.neg;
-    b.then([=](hipStream_t st) { o->seg_finish(st, mod, neg, nullptr, (long)segs, 1, nz); });
+    b.then([=](hipStream_t st) { o->seg_finish(st, mod, neg, (long)segs, nz); });
     plan_invert(b, key, nz, nullptr, false, (long)segs, nz);
     b.then([=](hipStream_t st) { o->mont_times_z(st, mod, pos, nz, (long)segs, out); });
   }

@@ -454,7 +454,6 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
     l.neg_off = dot ? sc.add(l.ch.size() * NLB) : 0;
   }
   size_t tab_off = dot ? sc.add(lv[0].ch.size() * (NLB << heu_hip::kPowWindowBits)) : 0;
-  size_t nt_off = sc.add(segments * 8);
   size_t top_pos = sc.add(segments * NLB), top_neg = dot ? sc.add(segments * NLB) : 0;
   size_t nz_off = dot ? sc.add(segments * (size_t)key->ct_words * 4) : 0;
   size_t inv_base = dot ? sc.add(inv_scratch_bytes(key, (long)segments)) : 0;
@@ -468,7 +467,7 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
                    sc.at<uint32_t>(lv[0].neg_off), sc.at<uint32_t>(tab_off), idx));
   else
     LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, !mform, ct, sc.at<Chunk>(lv[0].ch_off), (long)lv[0].ch.size(),
-                                            sc.at<uint32_t>(lv[0].pos_off), idx, mform ? mod.one : nullptr));
+                                            sc.at<uint32_t>(lv[0].pos_off), idx));
   for (size_t i = 1; i < lv.size(); ++i) {
     if (dot) {
       LAUNCH(ctx, "reduce_mont", o->reduce_mont(st, mod, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off), (long)lv[i].ch.size(),
@@ -477,7 +476,7 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
                      sc.at<uint32_t>(lv[i].neg_off)));
     } else {
       LAUNCH(ctx, "reduce_sum", o->reduce_sum(st, mod, false, sc.at<uint32_t>(lv[i - 1].pos_off), sc.at<Chunk>(lv[i].ch_off),
-                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off), nullptr, nullptr));
+                    (long)lv[i].ch.size(), sc.at<uint32_t>(lv[i].pos_off), nullptr));
     }
   }
   // the last level has one chunk per segment, in segment order
@@ -489,15 +488,13 @@ int segmented(heu_key* key, const uint32_t* ct, const int64_t* w, const int64_t*
   }
   HIP_TRY(hipMemcpyAsync(sc.at<char>(top_pos), sc.at<char>(top.pos_off), segments * NLB, hipMemcpyDeviceToDevice, st));
   if (!dot) {
-    std::vector<int64_t> nt(len.begin(), len.end());
-    if ((rc = upload_meta(ctx, sc, nt_off, nt.data(), segments * 8))) return rc;
-    LAUNCH(ctx, "seg_finish", o->seg_finish(st, mod, sc.at<uint32_t>(top_pos), sc.at<int64_t>(nt_off), (long)segments, 0, out));
+    LAUNCH(ctx, "seg_finish", o->seg_finish(st, mod, sc.at<uint32_t>(top_pos), (long)segments, out));
   } else {
     HIP_TRY(
         hipMemcpyAsync(sc.at<char>(top_neg), sc.at<char>(top.neg_off), segments * NLB, hipMemcpyDeviceToDevice, st));
     // N -> Z-space, invert all N in one batch, result = P * N^-1
     uint32_t* nz = sc.at<uint32_t>(nz_off);
-    LAUNCH(ctx, "seg_finish", o->seg_finish(st, mod, sc.at<uint32_t>(top_neg), nullptr, (long)segments, 1, nz));
+    LAUNCH(ctx, "seg_finish", o->seg_finish(st, mod, sc.at<uint32_t>(top_neg), (long)segments, nz));
     if ((rc = batch_invert(key, nz, nullptr, (long)segments, nz, inv_base))) return rc;
     LAUNCH(ctx, "mont_times_z", o->mont_times_z(st, mod, sc.at<uint32_t>(top_pos), nz, (long)segments, out));
   }
@@ -627,6 +624,21 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
   size_t* tb = &key->device_bytes;
   int rc = upload_mod(n2, ops->n_L, ops->n_K, key->n2_m, key->n2_one, key->n2_r2, nullptr, &key->n2_mprime, st, tb, bc.c);
   if (rc) return rc;
+  {
+    // R^(k+1) mod n^2 for k < kRpowEntries: the Montgomery forms of R^k that
+    // bring a chunk of k Z-space terms' Montgomery product into Montgomery form
+    Bn R, r;
+    BN_set_bit(R, heu_hip::kLimbBits * key->nl_n);
+    BN_nnmod(r, R, n2, bc.c);
+    std::vector<uint32_t> tab;
+    tab.reserve((size_t)heu_hip::kRpowEntries * key->nl_n);
+    for (int k = 0; k < heu_hip::kRpowEntries; ++k) {
+      const std::vector<uint32_t> e = r29_interleaved(r, ops->n_L, ops->n_K);
+      tab.insert(tab.end(), e.begin(), e.end());
+      BN_mod_mul(r, r, R, n2, bc.c);
+    }
+    if ((rc = upload(key->n2_rpow, tab, st, tb))) return rc;
+  }
   if ((rc = upload(key->n29, r29_plain(n, key->nl_n), st, tb))) return rc;
   key->n2_host = bn_to_words(n2, key->ct_words);
   if ((rc = upload(key->n2_words, key->n2_host, st, tb))) return rc;
@@ -1055,7 +1067,7 @@ int heu_from_mform(heu_key* key, const uint32_t* mf, size_t count, uint32_t* out
   if (count == 0) return HEU_OK;
   if (!mf || !out) return fail(HEU_EINVAL, "null argument");
   // x R -> x, canonical words: the Montgomery-input finish of a segment product
-  LAUNCH(ctx, "from_mform", o->seg_finish(ctx->stream, key->mod_n2(), mf, nullptr, (long)count, 1, out));
+  LAUNCH(ctx, "from_mform", o->seg_finish(ctx->stream, key->mod_n2(), mf, (long)count, out));
   HIP_TRY(hipGetLastError());
   return HEU_OK;
 }

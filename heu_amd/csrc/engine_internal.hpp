@@ -74,6 +74,7 @@ std::vector<long> inv_levels(long count, long chunk0);
 // idles: chunks of kReduceChunkUpper keep that chain short (128 partials per
 // segment: 4 + 4 + 4 + 2 products instead of 32 + 4).
 constexpr long kReduceChunk0 = 32, kReduceChunkDot0 = 8, kReduceChunkUpper = 4;
+static_assert(kReduceChunk0 < heu_hip::kRpowEntries, "a Z-space level-0 chunk needs its R power");
 // Level 0's chunk for a reduction of `total` terms: the full chunk when the
 // terms fill the chip about twice over, shorter for fewer, so a short vector's
 // level 0 is a short chain of products rather than a long chain on a few groups.
@@ -202,7 +203,7 @@ struct heu_key {
   int nl_n = 0;
   uint32_t r_bits = 0;  // randomness bits the table covers
   // public side
-  heu_engine::DevBuf n2_m, n2_one, n2_r2, n29, table, n2_words;
+  heu_engine::DevBuf n2_m, n2_one, n2_r2, n2_rpow, n29, table, n2_words;
   uint32_t n2_mprime = 0;
   int windows = 0, unit = 0;
   // secret side: p^2 / q^2 constants for every decryption shape, one active
@@ -224,7 +225,8 @@ struct heu_key {
   size_t device_bytes = 0;
 
   ModArgs mod_n2() const {
-    return ModArgs{n2_m.as<uint32_t>(), n2_one.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_mprime};
+    return ModArgs{n2_m.as<uint32_t>(), n2_one.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_r2.as<uint32_t>(), n2_mprime,
+                   n2_rpow.as<uint32_t>()};
   }
   PubArgs pub() const { return PubArgs{mod_n2(), n29.as<uint32_t>(), table.as<uint32_t>(), windows, unit}; }
   const heu_hip::DecShape& dshape() const { return ops->dec[dec_shape]; }

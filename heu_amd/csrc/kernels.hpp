@@ -19,7 +19,12 @@ struct ModArgs {
   const uint32_t* r2;   // R^2 mod m
   const uint32_t* r3;   // R^3 mod m (p^2 and q^2 only)
   uint32_t mprime;      // -m^-1 mod 2^29
+  // n^2 only: entry k (NL words, k = 0 .. kRpowEntries-1) is R^(k+1) mod m,
+  // the Montgomery form of R^k.  A Z-space chunk of k terms multiplied by its
+  // entry leaves the chunk's product in Montgomery form.
+  const uint32_t* rpow;
 };
+constexpr int kRpowEntries = 33;  // chunks of up to 32 Z-space terms
 
 // Public-key side, modulus n^2.
 struct PubArgs {
@@ -95,15 +100,16 @@ typedef void (*AddPtFn)(hipStream_t, const PubArgs&, const uint32_t*, const int6
 typedef void (*PowFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, uint32_t*, u8*, uint32_t*);
 typedef void (*PowWordsFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, const u8*, int, long,
                            uint32_t*, u8*, uint32_t*);
-// zin: Z-space words at level 0 (else R29 limbs); mont_one (limbs only): the
-// value of an empty chunk is R mod m instead of 1 (engine-form sums)
+// zin: Z-space words at level 0 (else R29 Montgomery limbs); either way each
+// partial is the Montgomery form of its chunk's product (an empty chunk: of 1)
 typedef void (*ReduceSumFn)(hipStream_t, const ModArgs&, bool, const uint32_t*, const Chunk*, long, uint32_t*,
-                            const int64_t*, const uint32_t*);
+                            const int64_t*);
 typedef void (*ReduceDot0Fn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, const Chunk*, long,
                              uint32_t*, uint32_t*, uint32_t*, const int64_t*);
 typedef void (*PrefixMulFn)(hipStream_t, const ModArgs&, const uint32_t*, const Chunk*, long, uint32_t*);
 typedef void (*ReduceMontFn)(hipStream_t, const ModArgs&, const uint32_t*, const Chunk*, long, uint32_t*);
-typedef void (*SegFinishFn)(hipStream_t, const ModArgs&, const uint32_t*, const int64_t*, long, int, uint32_t*);
+// Montgomery-form values -> canonical Z-space words
+typedef void (*SegFinishFn)(hipStream_t, const ModArgs&, const uint32_t*, long, uint32_t*);
 typedef void (*MontTimesZFn)(hipStream_t, const ModArgs&, const uint32_t*, const uint32_t*, long, uint32_t*);
 // Which level-0 elements a batch inversion works on: all of them (both null),
 // those with sel[i] != 0, or positions [0, *nsel) of the index list idx.

@@ -785,8 +785,11 @@ __global__ __launch_bounds__(kBlock) void k_dec_combine(CombArgs C, const u32* _
 // ---------------------------------------------------------------------------
 // Segment reductions (numpy/evaluator.cc:355-373 Sum, :209-292 MatMul dot).
 // A chunk is a contiguous run of one segment; one group reduces one chunk.
-//  - sum: Montgomery products of Z-space inputs; a product of N terms through
-//    N-1 multiplications carries R^-(N-1), removed by k_seg_finish.
+//  - sum: Montgomery products.  Z-space inputs enter as they are, so a chunk's
+//    N terms through N-1 products carry R^-(N-1); one product by the
+//    Montgomery form of R^N (mod.rpow) leaves every partial in Montgomery
+//    form, the form engine-form inputs have already.  Upper levels multiply
+//    partials, and k_seg_finish brings each segment's total back to Z-space.
 //  - dot (level 0): term = (c R)^|w| in Montgomery space, multiplied into a
 //    positive or negative accumulator by the sign of w (w = 0 contributes 1).
 // Partials are R29 interleaved.
@@ -795,18 +798,17 @@ __global__ __launch_bounds__(kBlock) void k_dec_combine(CombArgs C, const u32* _
 // (gathered sums: selections, bucket sums), else element j.
 __device__ __forceinline__ long at_pos(const i64* __restrict__ idx, long j) { return idx ? (long)idx[j] : j; }
 
-// An empty chunk gives 1; in engine form (mont_one != null, the R29 Montgomery
-// form of 1) it gives R mod m, so that a sum of engine-form values stays in
-// engine form.
+// Each partial is the Montgomery form of its chunk's product; an empty chunk
+// gives R mod m, the Montgomery form of 1.  Z-space chunks hold at most
+// kRpowEntries - 1 terms (the host planners' level-0 chunk).
 template <int L, int K, int W, bool ZIN>
-__device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const u32* __restrict__ in, const Chunk ch,
-                                                u32* __restrict__ part, const i64* __restrict__ idx = nullptr,
-                                                const u32* __restrict__ mont_one = nullptr) {
+__device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const ModArgs& mod, const u32* __restrict__ in,
+                                                const Chunk ch, u32* __restrict__ part,
+                                                const i64* __restrict__ idx = nullptr) {
   constexpr int NL = L * K;
   u32 x[K];
   if (ch.len == 0) {
-    if (mont_one) load_r29<L, K>(x, mont_one, g.lid);
-    else one_limbs<L, K>(x, g.lid);
+    load_r29<L, K>(x, mod.one, g.lid);
     store_r29<L, K>(part, x, g.lid);
     return;
   }
@@ -818,6 +820,7 @@ __device__ __forceinline__ void reduce_sum_body(Grp<L, K, W>& g, const u32* __re
     else load_r29<L, K>(x, in + i * NL, g.lid);
     g.mul(x);
   }
+  if constexpr (ZIN) g.mul_global(mod.rpow + ch.len * NL);  // (prod R^-(len-1)) R^(len+1) R^-1
   g.save(part, g.A);
 }
 
@@ -825,13 +828,12 @@ template <int L, int K, int W, bool ZIN>
 __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_sum(ModArgs mod, const u32* __restrict__ in,
                                                                              const Chunk* __restrict__ chunks,
                                                                              long nchunks, u32* __restrict__ part,
-                                                                             const i64* __restrict__ idx,
-                                                                             const u32* __restrict__ mont_one) {
+                                                                             const i64* __restrict__ idx) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, mod, e);
-  if (e < nchunks) reduce_sum_body<L, K, W, ZIN>(g, in, chunks[e], part + e * NL, idx, mont_one);
+  if (e < nchunks) reduce_sum_body<L, K, W, ZIN>(g, mod, in, chunks[e], part + e * NL, idx);
 }
 
 template <int L, int K, int W>
@@ -894,36 +896,18 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_reduce_mont(
   g.save(part + e * NL, g.A);
 }
 
-// Finish a segment sum: the total carries R^-(N-1); multiply by the
-// Montgomery form of R^(N-1), i.e. R^N, built as a Montgomery power of R^2
-// (the Montgomery form of R).  With mont_in set, the partial is a
-// Montgomery-form value and is only mapped back to Z-space.
+// Finish a segment sum (or a dot's negative product, or engine-form values
+// leaving the engine): Montgomery form -> canonical Z-space words.
 template <int L, int K, int W>
-__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_seg_finish(ModArgs mod, const u32* __restrict__ part, const i64* __restrict__ n_terms,
-                                   long nseg, int mont_in, u32* __restrict__ out) {
+__global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_seg_finish(ModArgs mod, const u32* __restrict__ part,
+                                                                             long nseg, u32* __restrict__ out) {
   constexpr int NL = L * K;
   LDS_DECL(L, K, W);
   long e;
   auto g = grp_setup<L, K, W>(lds, mod, e);
   if (e >= nseg) return;
-  if (mont_in) {
-    g.set_global(g.A, part + e * NL);
-    g.from_mont();
-  } else {
-    const i64 nt = n_terms[e];
-    const u64 ex = nt > 1 ? (u64)(nt - 1) : 0u;
-    g.set_global(g.A, mod.one);
-    int top = 63;
-    while (top > 0 && ((ex >> top) & 1u) == 0) --top;
-    for (int bit = top; bit >= 0; --bit) {
-      g.square();
-      if ((ex >> bit) & 1u) g.mul_global(mod.r2);
-    }
-    u32 x[K];
-    get_lds<L, K>(x, g.A, g.lid);
-    g.set_global(g.A, part + e * NL);
-    g.mul(x);  // total * R^(ex+1) * R^-1
-  }
+  g.set_global(g.A, part + e * NL);
+  g.from_mont();
   g.store_canonical(out + e * W, W);
 }
 
@@ -1558,8 +1542,9 @@ __global__ __launch_bounds__(kBlock, (Cfg<L, K, W>::kWaves)) void k_worklist(con
                              static_cast<u32*>(it.out2) + e * NL, T, static_cast<const i64*>(it.d));
             break;
           case kWorkSum0:
-            reduce_sum_body<L, K, W, true>(g, static_cast<const u32*>(it.a), static_cast<const Chunk*>(it.c)[e],
-                                           static_cast<u32*>(it.out) + e * NL, static_cast<const i64*>(it.d));
+            reduce_sum_body<L, K, W, true>(g, it.pub.n2, static_cast<const u32*>(it.a),
+                                           static_cast<const Chunk*>(it.c)[e], static_cast<u32*>(it.out) + e * NL,
+                                           static_cast<const i64*>(it.d));
             break;
           default:
             break;
@@ -1631,9 +1616,9 @@ struct Launch {
     HEU_LAUNCH((k_pow_words<L, K, W>), n, L, st, mod, ct, kmag, kneg, kw, n, out, neg, tab);
   }
   static void reduce_sum(hipStream_t st, const ModArgs& mod, bool zin, const u32* in, const Chunk* chunks, long nch,
-                         u32* part, const i64* idx, const u32* mont_one) {
-    if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part, idx, nullptr);
-    else HEU_LAUNCH((k_reduce_sum<L, K, W, false>), nch, L, st, mod, in, chunks, nch, part, nullptr, mont_one);
+                         u32* part, const i64* idx) {
+    if (zin) HEU_LAUNCH((k_reduce_sum<L, K, W, true>), nch, L, st, mod, in, chunks, nch, part, idx);
+    else HEU_LAUNCH((k_reduce_sum<L, K, W, false>), nch, L, st, mod, in, chunks, nch, part, nullptr);
   }
   static void to_mform(hipStream_t st, const ModArgs& mod, const u32* ct, long n, u32* out) {
     HEU_LAUNCH((k_to_mform<L, K, W>), n, L, st, mod, ct, n, out);
@@ -1652,9 +1637,8 @@ struct Launch {
                           u32* part) {
     HEU_LAUNCH((k_reduce_mont<L, K, W>), nch, L, st, mod, in, chunks, nch, part);
   }
-  static void seg_finish(hipStream_t st, const ModArgs& mod, const u32* part, const i64* nt, long nseg, int mont_in,
-                         u32* out) {
-    HEU_LAUNCH((k_seg_finish<L, K, W>), nseg, L, st, mod, part, nt, nseg, mont_in, out);
+  static void seg_finish(hipStream_t st, const ModArgs& mod, const u32* part, long nseg, u32* out) {
+    HEU_LAUNCH((k_seg_finish<L, K, W>), nseg, L, st, mod, part, nseg, out);
   }
   static void mont_times_z(hipStream_t st, const ModArgs& mod, const u32* xm, const u32* yz, long n, u32* out) {
     HEU_LAUNCH((k_mont_times_z<L, K, W>), n, L, st, mod, xm, yz, n, out);
