@@ -404,9 +404,13 @@ __device__ __forceinline__ void canonicalize(u32 (&x)[K], const GMod<L, K>& md) 
 
 // 32-bit little-endian words staged in LDS -> this lane's limbs starting at
 // limb `first` (limbs past the number are 0).
-template <int L, int K>
+// kFresh hides the lane index from the optimizer, so that the K word offsets
+// are computed here, at each use, instead of being hoisted out of a caller's
+// loop, where they would stay live across the Montgomery products and spill.
+template <int L, int K, bool kFresh = false>
 __device__ __forceinline__ void words_to_limbs(const u32* __restrict__ w_lds, int words, int first, u32 (&x)[K]) {
-  const u32 lid = grp_lane<L>();
+  u32 lid = grp_lane<L>();
+  if constexpr (kFresh) asm volatile("" : "+v"(lid));
 #pragma unroll
   for (int t = 0; t < K; ++t) {
     int j = first + (int)lid * K + t;

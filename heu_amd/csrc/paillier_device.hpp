@@ -99,7 +99,7 @@ struct Grp {
   __device__ __forceinline__ void load_words(u32 (&x)[K], const u32* __restrict__ src, int words, int first = 0) {
     u32* st = words <= C::S ? B : A;
     stage_words<L>(st, src, words);
-    words_to_limbs<L, K>(st, words, first, x);
+    words_to_limbs<L, K, true>(st, words, first, x);
   }
   // Canonical residue of A written as `words` 32-bit words.
   __device__ __forceinline__ void store_canonical(u32* __restrict__ dst, int words) {
