@@ -370,3 +370,52 @@ def test_chain_full_size(eng, ctx, k2048):
     gk.ctx.sync()
     assert np.array_equal(host(d), chain_expected(m, m[::-1], k))
     assert np.array_equal(host(c1[:4096]), ok.encrypt_i64(m[:4096], r[:4096], threads=8))
+
+
+@pytest.mark.parametrize("which", ["k2048", "k3072"])
+def test_sum_every_chunk_length(eng, ctx, which, request):
+    # Z-space sums over enough terms (> 31 x 32,768) that level 0 takes chunks
+    # of 32, with segments whose last chunk holds each length 0..31: every
+    # entry of the per-chunk R-power table is used.  Each segment's total must
+    # decrypt to the plaintext sum; three segments are also bit-exact against
+    # the oracle.  The same segments run as a gathered sum (a permuted index
+    # list) and as a batch task (work-list kernel).
+    kd, gk, ok = request.getfixturevalue(which)
+    lens = [32 * 3 + r for r in range(32)] + [0, 1, 31, 33]
+    lens.append(1_040_000 - sum(lens))
+    offsets = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(offsets[-1])
+    rng = np.random.default_rng(31)
+    m = rng.integers(-(2**40), 2**40, size=n, dtype=np.int64)
+    r = eng.seeded_r(31, 11, n, 256)
+    print(f"[{which}] sum of {n} terms in {len(lens)} segments: encrypt", flush=True)
+    ct = gk.encrypt_i64(m, r)
+    want = [int(m[a:b].sum()) for a, b in zip(offsets[:-1], offsets[1:])]
+
+    def check(s, what):
+        d = gk.decrypt(s, out_words=4)
+        gk.ctx.sync()
+        got = eng.words_to_ints(host(d), signed=True)
+        assert got == want, what
+
+    print("one-op sum", flush=True)
+    s = gk.sum(ct, offsets)
+    check(s, "one-op sum")
+    hs = host(s)
+    for j in (0, 17, 31):
+        a, b = offsets[j], offsets[j + 1]
+        assert np.array_equal(hs[j : j + 1], ok.dot_i64(host(ct[a:b]), None, np.array([0, b - a], np.int64))), j
+    print("gathered sum", flush=True)
+    perm = rng.permutation(n).astype(np.int64)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(n, dtype=np.int64)
+    g = gk.sum_gather(ct[torch.from_numpy(perm).to(ct.device)], torch.from_numpy(inv).to(ct.device), offsets)
+    check(g, "gathered sum")
+    assert np.array_equal(host(g), hs)
+    print("batch sum", flush=True)
+    b = eng.Batch(ctx, graph=False)
+    bs = b.sum(gk, ct, offsets)
+    b.run()
+    ctx.sync()
+    assert np.array_equal(host(bs), hs)
+    b.close()
