@@ -96,7 +96,7 @@ long reduce_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, bool dot, long total) {
 
 long inv_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, long count) {
   const long target = fill_groups(ctx, o);
-  return std::max(kInvChunkUpper, std::min(kInvChunk0, (count + target - 1) / target));
+  return std::max(kInvChunk0Min, std::min(kInvChunk0, (count + target - 1) / target));
 }
 
 std::vector<heu_hip::Chunk> make_chunks(const std::vector<long>& seg_start, const std::vector<long>& seg_len, long chunk,

@@ -56,11 +56,11 @@ struct DevBuf {
 // Batch inversion (Montgomery's trick): level 0 multiplies the inputs in
 // chunks of up to kInvChunk0, which fills the chip for large batches.  A batch
 // that would not fill the chip about twice over gets shorter level-0 chunks
-// (inv_chunk0, at least kInvChunkUpper), so a short batch is a short chain of
+// (inv_chunk0, at least kInvChunk0Min), so a short batch is a short chain of
 // products rather than a long chain on a few groups.  Upper levels see few
 // values, where a level's time is its chunk's chain of sequential products, so
-// they use chunks of kInvChunkUpper.
-constexpr long kInvChunk0 = 32, kInvChunkUpper = 4;
+// they multiply pairs (kInvChunkUpper), as the reductions' upper levels do.
+constexpr long kInvChunk0 = 32, kInvChunk0Min = 4, kInvChunkUpper = 2;
 long inv_chunk0(heu_ctx* ctx, const heu_hip::Ops* o, long count);
 inline int inv_chunk(size_t level, long chunk0) { return (int)(level == 0 ? chunk0 : kInvChunkUpper); }
 
@@ -71,9 +71,11 @@ std::vector<long> inv_levels(long count, long chunk0);
 // chunks of up to kReduceChunk0 (kReduceChunkDot0 for dot terms, each a power),
 // which fills the chip for large inputs.  Upper levels see few partials, so a
 // level takes as long as its chunk's sequential products while most of the chip
-// idles: chunks of kReduceChunkUpper keep that chain short (128 partials per
-// segment: 4 + 4 + 4 + 2 products instead of 32 + 4).
-constexpr long kReduceChunk0 = 32, kReduceChunkDot0 = 8, kReduceChunkUpper = 4;
+// idles, and a level costs about one product's latency per product in its
+// chain plus a launch: pairs (kReduceChunkUpper = 2) take 128 partials per
+// segment to one in 7 levels of one product each, 214 us for a 4096 x 256
+// axis-0 sum against 257 us in 4 levels of chunks of 4 (3 + 3 + 3 + 1).
+constexpr long kReduceChunk0 = 32, kReduceChunkDot0 = 8, kReduceChunkUpper = 2;
 static_assert(kReduceChunk0 < heu_hip::kRpowEntries, "a Z-space level-0 chunk needs its R power");
 // Level 0's chunk for a reduction of `total` terms: the full chunk when the
 // terms fill the chip about twice over, shorter for fewer, so a short vector's
