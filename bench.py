@@ -221,6 +221,81 @@ def spi_span_rates(count: int = 65536) -> dict:
     return out
 
 
+def cpu_config_samples(threads: int, seed: int = 0) -> dict:
+    """The oracle timed on samples of configs 3 and 4's own inputs (the same
+    seeded keys and data streams as heu_amd/configs.py), on every usable core:
+    seconds per element for each op and key size, and the config's CPU time
+    scaled from them.  Dots and sums run as `threads` contiguous pieces in
+    parallel (the oracle's product loop is single-threaded)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from heu_amd import configs as C
+    from heu_amd import engine as E
+    from oracle import oracle as O
+
+    def okey(bits, kseed):
+        kd = E.keygen(bits, kseed)
+        k = O.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
+        k.build_table(10, 256)
+        return k
+
+    def timed(fn):
+        t = time.perf_counter()
+        fn()
+        return time.perf_counter() - t
+
+    def par_dot(key, cts, w):
+        cuts = np.linspace(0, len(cts), threads + 1).astype(np.int64)
+
+        def piece(i):
+            a, b = int(cuts[i]), int(cuts[i + 1])
+            return key.dot_i64(cts[a:b], None if w is None else w[a:b], np.array([0, b - a], np.int64))
+
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(piece, range(threads)))
+
+    out = {}
+    # config 3: the first terms of the 64 vectors' concatenation (3072-bit key)
+    key = okey(3072, (3 << 16) | seed)
+    total3 = int(C.log_uniform_lengths(3, seed, 1, 64, 1, 65536).sum())
+    s = min(8192, total3)
+    m, r, w = C.int64s(3, seed, 2, s), C.r_limbs(3, seed, 3, s), C.int32s(3, seed, 4, s)
+    holder = {}
+    t_enc = timed(lambda: holder.update(ct=key.encrypt_i64(m, r, use_table=True, threads=threads)))
+    t_dot = timed(lambda: par_dot(key, holder["ct"], w))
+    per = {"enc_3072": t_enc / s, "dot_3072": t_dot / s}
+    out["3"] = {"sample_terms": s, "seconds_per_element": per,
+                "cpu_all_core_s_sampled": total3 * (per["enc_3072"] + per["dot_3072"])}
+    # config 4: vector 0 (2048-bit key 0) and vector 8 (3072-bit key 8), each op
+    # on the first elements of its 40/20/20/20 range, plus a piece of the
+    # 4096 x 256 axis-0 sum on key 0
+    lens4 = C.log_uniform_lengths(4, seed, 1, 16, 256, 262144)
+    per4, samples = {}, {}
+    for i in (0, 8):
+        bits = 2048 if i < 8 else 3072
+        key = okey(bits, (4 << 16) | (seed << 8) | i)
+        n = int(lens4[i])
+        cut = [0, int(0.4 * n), int(0.6 * n), int(0.8 * n), n]
+        m, r, k = C.int64s(4, seed, 100 + i, n), C.r_limbs(4, seed, 200 + i, n), C.int32s(4, seed, 300 + i, n)
+        x = min(2048, cut[1] - cut[0], cut[2] - cut[1], cut[3] - cut[2], cut[4] - cut[3])
+        samples[bits] = x
+        ct = key.encrypt_i64(m, r, use_table=True, threads=threads)
+        ct2 = key.encrypt_i64(m[::-1].copy(), r, use_table=True, threads=threads)
+        a0, a1, a2, a3 = cut[:4]
+        per4[f"enc_{bits}"] = timed(lambda: key.encrypt_i64(m[a0:a0 + x], r[a0:a0 + x], use_table=True,
+                                                            threads=threads)) / x
+        per4[f"dec_{bits}"] = timed(lambda: key.decrypt(ct[a1:a1 + x], out_words=2, threads=threads)) / x
+        per4[f"add_{bits}"] = timed(lambda: key.add(ct[a2:a2 + x], ct2[a2:a2 + x], threads=threads)) / x
+        per4[f"mul_{bits}"] = timed(lambda: key.mul_i64(ct[a3:a3 + x], k[a3:a3 + x], threads=threads)) / x
+        if bits == 2048:
+            rows = 4096
+            mat = key.encrypt_i64(C.int64s(4, seed, 400, 2 * rows) >> 16, C.r_limbs(4, seed, 401, 2 * rows),
+                                  use_table=True, threads=threads)
+            per4["sum_2048"] = timed(lambda: par_dot(key, mat, None)) / (2 * rows)
+    out["4"] = {"sample_elements_per_op": samples, "seconds_per_element": per4}
+    return out
+
+
 def cpu_work_seconds(work: dict, cpu: dict) -> float:
     """CPU seconds for a workload given as {op: {key_bits: elements}}, at the
     oracle's all-core rates: a dot term is a ct x int32 and a ct+ct (numpy
@@ -461,6 +536,7 @@ def main():
 
         del key, m, r, k, ct_res, out
         torch.cuda.empty_cache()
+        samples = cpu_config_samples(cpu["cores"]) if cpu and ("3" in wanted or "4" in wanted) else {}
         for c in wanted:
             t = time.perf_counter()
             print(f"[bench] config {c} ...", file=sys.stderr, flush=True)
@@ -470,6 +546,15 @@ def main():
                 secs = cpu_work_seconds(extra[c]["work"], cpu)
                 extra[c]["cpu_all_core_s"] = secs
                 extra[c]["cpu_cores"] = cpu["cores"]
+                if c in samples:
+                    # the same workload timed on samples of its own keys and inputs
+                    smp = dict(samples[c])
+                    if "cpu_all_core_s_sampled" not in smp:
+                        per = smp["seconds_per_element"]
+                        smp["cpu_all_core_s_sampled"] = sum(
+                            cnt * per[f"{op}_{bits}"] for op, by in extra[c]["work"].items() for bits, cnt in by.items()
+                            if cnt)
+                    extra[c]["cpu_sample"] = smp
                 if "batch_graph_ms" in extra[c]:
                     extra[c]["speedup_batch_vs_cpu"] = secs / (extra[c]["batch_graph_ms"] / 1e3)
                     extra[c]["speedup_plain_vs_cpu"] = secs / (extra[c]["plain_ms"] / 1e3)

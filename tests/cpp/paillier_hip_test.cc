@@ -393,7 +393,9 @@ void VectorizedClosedForms(const Keys& k, size_t count) {
 // closed forms, checked through decryption, and an error names the first bad
 // element in span order, as a serial loop would.
 void SpansSplitOverThreads(const Keys& k) {
-  const size_t count = 20000;
+  // 50,000 elements: split over threads on the host and into three pipelined
+  // pieces of about 16,667 on the device
+  const size_t count = 50000;
   Encryptor encryptor(k.pk);
   Evaluator evaluator(k.pk);
   Decryptor decryptor(k.pk, k.sk);
@@ -427,8 +429,9 @@ void SpansSplitOverThreads(const Keys& k) {
          dsub[i] == wrap(big[i] - small[i]);
   EXPECT_TRUE(ok);
 
-  // out-of-range plaintexts at 4100 (past the first chunk) and 19000 (in the
-  // last): the message names the one at 4100
+  // out-of-range plaintexts at 4100 (past the first host chunk) and 45000 (in
+  // the last piece): the message names the one at 4100; then at 20000 (the
+  // second piece, after the first ran) and 45000: it names the one at 20000
   auto names_first = [&](const std::function<void()>& call, const BigInt& first) {
     try {
       call();
@@ -439,17 +442,28 @@ void SpansSplitOverThreads(const Keys& k) {
   };
   std::vector<BigInt> bad = small;
   bad[4100] = k.pk.n_half_ + BigInt(7);
-  bad[19000] = k.pk.n_half_ + BigInt(9);
+  bad[45000] = k.pk.n_half_ + BigInt(9);
   auto pbad = cptrs(bad);
   EXPECT_TRUE(names_first([&] { encryptor.Encrypt(cs(pbad)); }, bad[4100]));
   EXPECT_TRUE(names_first([&] { evaluator.Add(cs(pcs), cs(pbad)); }, bad[4100]));
-  // a ciphertext >= n^2 deep in the span is rejected
+  bad[4100] = small[4100];
+  bad[20000] = k.pk.n_half_ + BigInt(11);
+  EXPECT_TRUE(names_first([&] { encryptor.Encrypt(cs(pbad)); }, bad[20000]));
+  EXPECT_TRUE(names_first([&] { evaluator.Add(cs(pcs), cs(pbad)); }, bad[20000]));
+  EXPECT_TRUE(names_first([&] { evaluator.Sub(cs(pcs), cs(pbad)); }, bad[20000]));
+  // a ciphertext >= n^2 deep in the span (the third piece) is rejected, and the
+  // calls after it still give every element right
   auto cbad = csm;
   std::vector<uint32_t> n2w(cbad[0].c_.size() + 1);
   k.pk.n_square_.ToWords(n2w.data(), n2w.size());
-  cbad[15000].c_.assign(n2w.begin(), n2w.end() - 1);
+  cbad[40000].c_.assign(n2w.begin(), n2w.end() - 1);
   EXPECT_THROW(evaluator.Add(cs(cptrs(cbad)), cs(pcs)));
+  EXPECT_THROW(evaluator.Negate(cs(cptrs(cbad))));
   EXPECT_THROW(decryptor.Decrypt(cs(cptrs(cbad))));
+  auto dagain = decryptor.Decrypt(cs(pcs));
+  bool again = dagain.size() == count;
+  for (size_t i = 0; again && i < count; ++i) again = dagain[i] == small[i];
+  EXPECT_TRUE(again);
 }
 
 // HEU's numpy layer calls the span methods from its worker threads, one chunk
@@ -653,7 +667,7 @@ int main() {
   run("KeyGenerator.Generate(3072)", [&] { KeyGenerator::Generate(3072, &k3072.sk, &k3072.pk); });
   run("ZPaillierTest.MinMaxDecrypt(3072)", [&] { MinMaxDecrypt(k3072); });
   run("Vectorized.ClosedForms(3072, 65)", [&] { VectorizedClosedForms(k3072, 65); });
-  run("Vectorized.SpansSplitOverThreads(2048, 20000)", [&] { SpansSplitOverThreads(seeded); });
+  run("Vectorized.SpansSplitOverThreads(2048, 50000)", [&] { SpansSplitOverThreads(seeded); });
   run("Concurrent chunks coalesce (2048 + 3072 keys)", [&] { ConcurrentChunks(seeded, k3072); });
   run("Vectorized.SpanRates(2048, 65536)", [&] { SpanRates(seeded, 65536); });
   run("Decryptor rejects an unpaired key", [&] { EXPECT_THROW(Decryptor(k2048.pk, k3072.sk)); });
