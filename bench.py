@@ -150,10 +150,12 @@ def cpu_rates_3072(threads: int) -> dict:
     return {"rates": rates, "sample": sizes}
 
 
-def cpu_baseline(kd, m, r, k):
-    """Times the oracle on this host: every op on the whole sample (65,536
-    elements of the same workload) over every usable core, plus each op's
-    single-core rate on a slice of it."""
+def cpu_baseline(kd, m, r, k, m_res):
+    """Times the oracle on this host running the chain itself on a sample of
+    its inputs (65,536 elements): encrypt, ct+ct with the resident operand
+    (Enc(m_res, r), encrypted untimed), ct x int32, decrypt, each over every
+    usable core; plus each op's single-core rate on a slice.  Returns the
+    oracle's output of every op, for the bit-exact comparison with the GPU's."""
     from oracle import oracle as O
 
     O.build()
@@ -161,19 +163,25 @@ def cpu_baseline(kd, m, r, k):
     key = O.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
     key.build_table(10, 256)
     n = len(m)
-    t = time.perf_counter()
-    ct = key.encrypt_i64(m, r, use_table=True, threads=threads)
-    rates = {"enc": n / (time.perf_counter() - t)}
-    rev = ct[::-1].copy()
+    res = key.encrypt_i64(m_res, r, use_table=True, threads=threads)
+    outs = {}
+
+    def timed(name, fn):
+        t = time.perf_counter()
+        outs[name] = fn()
+        rates[name] = n / (time.perf_counter() - t)
+
+    rates = {}
+    timed("enc", lambda: key.encrypt_i64(m, r, use_table=True, threads=threads))
+    timed("add", lambda: key.add(outs["enc"], res, threads=threads))
+    timed("ctpt", lambda: key.mul_i64(outs["add"], k, threads=threads))
+    timed("dec", lambda: key.decrypt(outs["ctpt"], out_words=4, threads=threads))
+    ct, rev = outs["enc"], outs["enc"][::-1].copy()
     ops = {
         "add": lambda x, th: key.add(ct[:x], rev[:x], threads=th),
         "ctpt": lambda x, th: key.mul_i64(ct[:x], k[:x], threads=th),
         "dec": lambda x, th: key.decrypt(ct[:x], out_words=2, threads=th),
     }
-    for name, fn in ops.items():
-        t = time.perf_counter()
-        fn(n, threads)
-        rates[name] = n / (time.perf_counter() - t)
     # single core: a slice sized to about a second per op at the multi-core rates
     single, single_n = {}, {}
     ops["enc"] = lambda x, th: key.encrypt_i64(m[:x], r[:x], use_table=True, threads=th)
@@ -185,14 +193,14 @@ def cpu_baseline(kd, m, r, k):
         single_n[name] = x
     chain = 1.0 / sum(1.0 / rates[x] for x in ("enc", "add", "ctpt", "dec"))
     r3072 = cpu_rates_3072(threads)
-    return ct, {
+    return outs, {
         "value": chain,
         "unit": "elements/s (enc+add+ct*pt+dec)",
         "cores": threads,
         "kind": "port",
         "host": host_info(),
-        "sample": f"{n} elements of the same workload per op (enc, add, ct*pt, dec), {threads} threads "
-        "(one contiguous chunk per thread per op); single-core rates on "
+        "sample": f"{n} elements of the chain itself (enc, add with the resident operand, ct*pt, dec of the "
+        f"product), {threads} threads (one contiguous chunk per thread per op); single-core rates on "
         + ", ".join(f"{x} {single_n[x]}" for x in single_n) + " elements; 3072-bit rates on "
         + ", ".join(f"{x} {v}" for x, v in r3072["sample"].items()) + " elements",
         "per_op": rates,
@@ -465,11 +473,13 @@ def main():
     cpu, parity = None, None
     if not args.no_cpu:
         n_cpu = min(N, 1 << 16)
-        ref_ct, cpu = cpu_baseline(kd, m_h[:n_cpu], r_h[:n_cpu], k_h[:n_cpu])
-        # the GPU's ciphertexts of the same inputs against the oracle's (paillier_z restated)
-        n_par = len(ref_ct)
-        gpu_ct = out[0][:n_par].cpu().numpy().view(np.uint32)
-        parity = {"elements": n_par, "encrypt_bit_exact": bool(np.array_equal(gpu_ct, ref_ct))}
+        # the resident operand's plaintexts: ct_res[i] = Enc(m[N-1-i], r[i])
+        ref, cpu = cpu_baseline(kd, m_h[:n_cpu], r_h[:n_cpu], k_h[:n_cpu], m_h[::-1][:n_cpu].copy())
+        # every op's output on these elements against the oracle's (paillier_z restated)
+        parity = {"elements": n_cpu}
+        for i, op in enumerate(("enc", "add", "ctpt", "dec")):
+            gpu = out[i][:n_cpu].cpu().numpy().view(np.uint32)
+            parity[f"{op}_bit_exact"] = bool(np.array_equal(gpu, ref[op]))
     line = {
         "metric": METRIC,
         "value": value,
