@@ -390,7 +390,7 @@ def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0, 60.0, 90
         key = E.Key.from_keygen(ctx, E.keygen(bits, (5 << 16) | (seed << 8) | i), r_bits=R_BITS, window_bits=10)
         pools.append(_Pool(key, seed, i, MAX_LEN5))
     ctx.sync()
-    srv = E.Server(ctx, workspace_bytes=24 << 30, max_inflight=2, urgent_ms=75)
+    srv = E.Server(ctx, workspace_bytes=24 << 30, max_inflight=2, urgent_ms=150)
     plain, batched, served = [], [], []
     work = {op: {2048: 0, 3072: 0} for op in OPS5}
     check_same = None

@@ -584,7 +584,7 @@ int heu_server_create(heu_ctx* ctx, const heu_server_config* cfg, heu_server** o
   if (!srv->cfg.max_batch_tasks) srv->cfg.max_batch_tasks = 512;
   if (!srv->cfg.batch_rounds) srv->cfg.batch_rounds = 1;
   if (!srv->cfg.parts_per_round) srv->cfg.parts_per_round = 8;
-  if (!srv->cfg.urgent_ms) srv->cfg.urgent_ms = 75;
+  if (!srv->cfg.urgent_ms) srv->cfg.urgent_ms = 150;
   if (srv->cfg.max_inflight > 8) return fail(HEU_EINVAL, "max_inflight must be at most 8");
   if (const char* f = std::getenv("HEU_SERVER_FAULT_BATCH")) srv->fault_batch = std::strtoull(f, nullptr, 10);
   HIP_TRY(hipSetDevice(ctx->device));

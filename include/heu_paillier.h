@@ -359,7 +359,7 @@ typedef struct heu_server_config {
                                units the device runs at once (0: 1) */
   uint32_t parts_per_round; /* elementwise requests are cut into parts of 1/parts_per_round of a round (0: 8) */
   uint32_t urgent_ms;       /* a request waiting longer than this goes ahead of every newer one and takes
-                               its shape's whole room, oldest first (0: 75; UINT32_MAX: never) */
+                               its shape's whole room, oldest first (0: 150; UINT32_MAX: never) */
   size_t max_workspace_bytes; /* a batch that does not fit the free workspace grows it by a new device
                                segment while hipMemGetInfo shows more than a reserve (5% of the device, at
                                least 2 GiB) free; the workspace stays within this (0: no other limit) */
