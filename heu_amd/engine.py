@@ -210,6 +210,8 @@ class Key:
             t = a.to(self.ctx.device)
         else:
             a = np.ascontiguousarray(a)
+            if not a.flags.writeable:  # a broadcast view: torch shares only writable memory
+                a = a.copy()
             if a.dtype == np.uint32:
                 a = a.view(np.int32)
             elif a.dtype == np.uint8:
