@@ -1,5 +1,5 @@
-"""numpy-style arrays of Paillier ciphertexts on the device: the HeMatrix
-surface of HEU's `heu.numpy` (heu/pylib/numpy_binding/bind_numpy.cc:321-510,
+"""Arrays of Paillier ciphertexts on the device: the HeMatrix surface of
+HEU's `heu.numpy` (heu/pylib/numpy_binding/bind_numpy.cc:321-510,
 heu/library/numpy/evaluator.h:24-92) for the Paillier path, over the engine.
 
   kit = hnp.setup(2048)                        # keys resident on the GPU
@@ -13,6 +13,15 @@ Plaintext arrays are numpy integer arrays (the reference wraps them in
 PlaintextArray).  Ciphertext arrays keep C order on the device; every
 reduction, selection and bucket sum runs as one gathered engine call
 (`heu_sum_gather` / `heu_dot_gather`), so no gathered copy is made.
+
+Ciphertext arrays index as HeMatrix does (pylib/numpy_binding/py_slicer.cc),
+which differs from numpy: a 2-tuple of row and column keys selects the block
+of their cross product (`x[[1, 2], [0, 3]]` is 2 x 2), and any other key
+selects rows over every column (`x[()]` selects nothing, `x[(5,)]` is a
+one-row matrix).  Keys are ints, slices and sequences of ints; 1-D boolean
+masks are also taken.  `select_sum` and `batch_select_sum` read keys the same
+way for plaintext and ciphertext arrays; plaintext arrays themselves index as
+numpy does.
 """
 from __future__ import annotations
 
@@ -65,15 +74,40 @@ class CiphertextArray:
         return self.shape[1] if self.ndim == 2 else 1
 
     def _flat_index(self, key) -> np.ndarray:
-        """Indices into `data` selected by a numpy-style key (ints, slices,
-        integer or boolean arrays, tuples of those)."""
-        grid = np.arange(self.size, dtype=np.int64).reshape(self.shape)
-        return np.asarray(grid[key], dtype=np.int64)
+        """Positions in `data` of the elements a HeMatrix key selects, in the
+        C order of the selected block."""
+        return _select_positions(self.shape, key)
 
     def __getitem__(self, key) -> "CiphertextArray":
-        sel = self._flat_index(key)
+        """HeMatrix indexing (pylib/numpy_binding/py_slicer.cc:58-86): a
+        2-tuple selects the block of its row and column indices, any other
+        key selects rows; an int index drops its axis."""
+        block, shape, transposed = _heu_block(self.shape, key)
+        sel = block.T if transposed else block
         idx = torch.from_numpy(np.ascontiguousarray(sel.ravel())).to(self.data.device)
-        return CiphertextArray(self.key, self.data.index_select(0, idx), sel.shape)
+        return CiphertextArray(self.key, self.data.index_select(0, idx), shape)
+
+    def __setitem__(self, key, value) -> None:
+        """HeMatrix assignment (py_slicer.cc:88-120, numpy/matrix.h:125-135):
+        the block the key selects (ints do not drop axes here) takes a
+        ciphertext array of the block's shape; a 1-D array fills a one-row
+        block.  Where an index repeats, the last write wins."""
+        if not isinstance(value, CiphertextArray):
+            raise TypeError(f"Unsupported value type [{type(value).__name__}] for __setitem__")
+        if value.key is not self.key:
+            raise ValueError("the assigned ciphertexts belong to another key")
+        block = _block_of(self.shape, key)[0]
+        pr, pc = value.rows(), value.cols()
+        if value.ndim == 1 and block.shape[0] == 1 and block.shape[1] > 1:
+            pr, pc = pc, pr
+        if (pr, pc) != block.shape:
+            raise ValueError(f"cannot assign a {pr} x {pc} array to a {block.shape[0]} x {block.shape[1]} block")
+        target = block.ravel()
+        # last write wins, as in the element-by-element assignment
+        _, first_from_end = np.unique(target[::-1], return_index=True)
+        src = (target.size - 1 - first_from_end).astype(np.int64)
+        dev = self.data.device
+        self.data[torch.from_numpy(target[src]).to(dev)] = value.data[torch.from_numpy(src).to(dev)]
 
     def transpose(self) -> "CiphertextArray":
         if self.ndim < 2:
@@ -145,6 +179,112 @@ class CiphertextArray:
     def _colmajor_of(rows: int, cols: int) -> np.ndarray:
         k = np.arange(rows * cols, dtype=np.int64)
         return (k % rows) * cols + k // rows
+
+
+# ---------------------------------------------------------------- HeMatrix keys
+def _checked_indices(a: np.ndarray, n: int) -> np.ndarray:
+    """slice_tool.cc:22-33: indices below n; a negative one counts from the end."""
+    a = a.astype(np.int64, copy=True)
+    hi = np.nonzero(a >= n)[0]
+    if hi.size:
+        raise IndexError(f"index {int(a[hi[0]])} is out of bounds [0, {n})")
+    a[a < 0] += n
+    lo = np.nonzero(a < 0)[0]
+    if lo.size:
+        raise IndexError(f"index {int(a[lo[0]]) - n} is out of bounds [{-n}, {n})")
+    return a
+
+
+def _axis_index(k, n: int):
+    """slice_tool::Parse (slice_tool.cc:35-87) against an axis of length n:
+    a slice, a sequence of ints (list, tuple, 1-D integer array) or an int
+    -> (indices, whether the axis is dropped).  A 1-D boolean mask array of
+    length n is also taken; the reference has no masks."""
+    if isinstance(k, slice):
+        return np.arange(*k.indices(n), dtype=np.int64), False
+    if isinstance(k, (int, np.integer)):
+        return _checked_indices(np.array([int(k)]), n), True
+    if isinstance(k, np.ndarray):
+        if k.dtype == np.bool_:
+            if k.ndim != 1 or k.shape[0] != n:
+                raise IndexError(f"a boolean index of shape {k.shape} does not match an axis of length {n}")
+            return np.nonzero(k)[0].astype(np.int64), False
+        if k.ndim != 1 or not np.issubdtype(k.dtype, np.integer):
+            raise TypeError(f"an index array must be 1-D of integers, got {k.ndim}-D {k.dtype}")
+        return _checked_indices(k, n), False
+    if isinstance(k, (list, tuple, range)):
+        if len(k) == 0:
+            return np.zeros(0, np.int64), False
+        for item in k:
+            if not isinstance(item, (int, np.integer)):
+                raise TypeError(f"indices array element must be integers, got {item!r} with type {type(item).__name__}")
+        return _checked_indices(np.array([int(i) for i in k]), n), False
+    raise TypeError(f"Unsupported indices type {type(k).__name__}")
+
+
+def _heu_key(key, ndim: int, rows: int, cols: int):
+    """py_slicer.cc:58-86 and extension_functions.cc:29-57: a 2-tuple indexes
+    rows and columns, and selects the block of their cross product; any other
+    key, a shorter tuple included, indexes rows over every column.
+    -> (row indices, column indices, drop rows, drop columns)."""
+    if isinstance(key, tuple):
+        if len(key) > ndim:
+            raise IndexError(f"too many indices for array, array is {ndim}-dimensional, but {len(key)} were indexed")
+        if len(key) == 2:
+            r, sr = _axis_index(key[0], rows)
+            c, sc = _axis_index(key[1], cols)
+            return r, c, sr, sc
+    r, sr = _axis_index(key, rows)
+    return r, np.arange(cols, dtype=np.int64), sr, False
+
+
+def _dims(shape):
+    """An array as the reference's rows x cols matrix (a vector is one
+    column, a scalar 1 x 1)."""
+    return (shape[0] if len(shape) else 1), (shape[1] if len(shape) == 2 else 1)
+
+
+def _block_of(shape, key):
+    rows, cols = _dims(shape)
+    r, c, sr, sc = _heu_key(key, len(shape), rows, cols)
+    return r[:, None] * cols + c[None, :], sr, sc
+
+
+def _select_positions(shape, key) -> np.ndarray:
+    """C-order positions of the block a key selects (no axis is dropped)."""
+    return _block_of(shape, key)[0].ravel()
+
+
+def _heu_block(shape, key):
+    """The block a key selects and the shape of the result, after
+    DenseMatrix::GetItem's squeeze rules (numpy/matrix.h:80-123).
+    -> (block of C-order positions, result shape, whether the block is read
+    transposed)."""
+    ndim = len(shape)
+    block, sq_row, sq_col = _block_of(shape, key)
+    if ndim == 1 and sq_col:
+        raise IndexError("axis doesn't exist, you cannot squeeze shape[1] of a vector")
+    if ndim == 0 and (sq_row or sq_col):
+        raise IndexError("axis doesn't exist, the array is 0-d")
+    nr, nc = block.shape
+    min_dim = 1 if (nr > 1 or nc > 1) else 0
+    new_dim, transposed = ndim, False
+    if (sq_row or sq_col) and ndim != min_dim:
+        if sq_col and nc <= 1:  # a column vector, or a scalar
+            new_dim = ndim - 1 - (1 if sq_row and nr <= 1 else 0)
+        elif sq_row and nr <= 1:  # a dropped row: the one-row block becomes a vector
+            new_dim, transposed = ndim - 1, True
+        else:
+            raise IndexError(f"cannot drop an axis of a {nr} x {nc} block")
+    if new_dim == 2:
+        out_shape = (nr, nc)
+    elif new_dim == 1:
+        out_shape = (block.size,)
+    elif block.size == 1:
+        out_shape = ()
+    else:
+        raise IndexError("a 0-d array has one element; this key selects none")
+    return block, out_shape, transposed
 
 
 def _is_ct(x) -> bool:
@@ -378,17 +518,21 @@ class Evaluator:
         return CiphertextArray(x.key, out, ())
 
     def select_sum(self, x, key):
-        """x[key].sum() without the copy (extension_functions.cc:29-57); an
-        empty selection gives an encryption of 0."""
+        """The sum of the elements a HeMatrix key selects, without a copy
+        (extension_functions.cc:29-57): a 2-tuple selects the block of its
+        row and column indices, any other key (a shorter tuple included)
+        selects rows.  An empty selection sums to 0 (an encryption of 0)."""
         if not _is_ct(x):
-            return np.sum(_pt(x)[key])
+            a = _pt(x)
+            return np.sum(a.ravel()[_select_positions(a.shape, key)])
         return self.batch_select_sum(x, [key]).reshape(())
 
     def batch_select_sum(self, x, keys):
-        """[x[k].sum() for k in keys] as one 1-D array (extension_functions.cc:59-87)."""
+        """[select_sum(x, k) for k in keys] as one 1-D array (extension_functions.cc:59-87)."""
         if not _is_ct(x):
-            return np.array([np.sum(_pt(x)[k]) for k in keys], dtype=np.int64)
-        sels = [x._flat_index(k).ravel() for k in keys]
+            a = _pt(x)
+            return np.array([np.sum(a.ravel()[_select_positions(a.shape, k)]) for k in keys], dtype=np.int64)
+        sels = [x._flat_index(k) for k in keys]
         offs = np.concatenate([[0], np.cumsum([len(s) for s in sels])]).astype(np.int64)
         idx = np.concatenate(sels).astype(np.int64) if sels else np.zeros(0, np.int64)
         if idx.size == 0:

@@ -74,12 +74,16 @@ def test_sums_and_selections(kit):
     X = enc.encrypt(x)
     print("hnp: sums", flush=True)
     assert dec.decrypt(ev.sum(X)) == x.sum()
+    # HeMatrix keys: a 2-tuple of index arrays selects the block of their cross
+    # product (pylib/numpy_binding/py_slicer.cc), unlike numpy's pairing
     keys = [(slice(None), 2), [1, 3, 5], x[:, 0] > 0, np.s_[2:6, 1:3], (np.array([0, 8]), np.array([3, 1])),
-            np.s_[0:0]]
-    for k in keys:
-        assert dec.decrypt(ev.select_sum(X, k)) == x[k].sum(), k
+            np.s_[0:0], ()]
+    want = [x[:, 2].sum(), x[[1, 3, 5]].sum(), x[x[:, 0] > 0].sum(), x[2:6, 1:3].sum(),
+            x[0, 3] + x[0, 1] + x[8, 3] + x[8, 1], 0, 0]
+    for k, w in zip(keys, want):
+        assert dec.decrypt(ev.select_sum(X, k)) == w, k
     got = dec.decrypt(ev.batch_select_sum(X, keys))
-    assert list(got) == [x[k].sum() for k in keys]
+    assert list(got) == want
     # bit-exact: a selection sum is the product of the selected ciphertexts mod n^2
     n2 = kit.key.n ** 2
     cts = [int.from_bytes(r.tobytes(), "little") for r in X.data.cpu().numpy().view(np.uint32)]
