@@ -291,11 +291,45 @@ def _is_ct(x) -> bool:
     return isinstance(x, CiphertextArray)
 
 
+_I64_MIN, _I64_MAX = -(2**63), 2**63 - 1
+
+
 def _pt(x) -> np.ndarray:
+    """A plaintext array: int64 when every value fits, else an object array of
+    Python ints (the reference's plaintexts are BigInts)."""
     a = np.asarray(x)
-    if a.dtype == object or not np.issubdtype(a.dtype, np.integer):
-        raise TypeError("plaintext arrays must hold integers that fit int64")
+    if a.dtype == object:
+        vals = a.ravel().tolist()
+        if not all(isinstance(v, (int, np.integer)) and not isinstance(v, (bool, np.bool_)) for v in vals):
+            raise TypeError("plaintext arrays must hold integers")
+        vals = [int(v) for v in vals]
+        if all(_I64_MIN <= v <= _I64_MAX for v in vals):
+            return np.array(vals, dtype=np.int64).reshape(a.shape)
+        return np.array(vals, dtype=object).reshape(a.shape)
+    if not np.issubdtype(a.dtype, np.integer):
+        raise TypeError("plaintext arrays must hold integers")
+    if a.dtype == np.uint64 and a.size and int(a.max()) > _I64_MAX:
+        return np.array([int(v) for v in a.ravel()], dtype=object).reshape(a.shape)
     return a.astype(np.int64)
+
+
+def _ints(a: np.ndarray) -> list:
+    return [int(v) for v in a.ravel()]
+
+
+def _negated(p: np.ndarray) -> np.ndarray:
+    """-p, as Python ints where int64 cannot hold it."""
+    if p.dtype == object or (p.size and int(p.min()) == _I64_MIN):
+        return np.array([-v for v in _ints(p)], dtype=object)
+    return -p
+
+
+def _add_pt(key: E.Key, cts: torch.Tensor, p: np.ndarray) -> torch.Tensor:
+    """Add(ct, p) = ct (1 + n p) mod n^2 (evaluator.cc:46-64): int64 plaintexts
+    in one kernel, wider ones as the encryption of p with r = 0, then ct+ct."""
+    if p.dtype == object:
+        return key.add(cts, key.encrypt(_ints(p), np.zeros((p.size, 1), np.uint32)))
+    return key.add_i64(cts, p)
 
 
 class Encryptor:
@@ -315,12 +349,16 @@ class Encryptor:
         return raw
 
     def encrypt(self, pts, seed=None) -> CiphertextArray:
+        """Integers of any size with |m| <= n/2 (encryptor.cc:35-58); int64
+        arrays take the int64 kernel."""
         a = _pt(pts)
         flat = np.ascontiguousarray(a.ravel())
+        key = self.kit.key
         if flat.size == 0:
-            return CiphertextArray(self.kit.key, self.kit.key._ct_out(0), a.shape)
-        ct = self.kit.key.encrypt_i64(flat, self._r(flat.size, seed))
-        return CiphertextArray(self.kit.key, ct, a.shape)
+            return CiphertextArray(key, key._ct_out(0), a.shape)
+        r = self._r(flat.size, seed)
+        ct = key.encrypt(_ints(flat), r) if flat.dtype == object else key.encrypt_i64(flat, r)
+        return CiphertextArray(key, ct, a.shape)
 
     def encrypt_with_audit(self, pts, seed=None):
         """(ciphertexts, audit strings) as numpy/encryptor.cc:107-121 and
@@ -333,7 +371,7 @@ class Encryptor:
         if flat.size == 0:
             return CiphertextArray(key, key._ct_out(0), a.shape), np.zeros(a.shape, dtype=object)
         r = self._r(flat.size, seed)
-        ct = key.encrypt_i64(flat, r)
+        ct = key.encrypt(_ints(flat), r) if flat.dtype == object else key.encrypt_i64(flat, r)
         rn = key.randomize(None, r)  # EncryptZero with the same r: h_s^r
         c_m = key.to_mont(ct, MONT_WORD_BITS).cpu().numpy().view(np.uint32)
         rn_m = key.to_mont(rn, MONT_WORD_BITS).cpu().numpy().view(np.uint32)
@@ -439,9 +477,9 @@ class Evaluator:
         if _is_ct(x) and _is_ct(y):
             out = key.add(self._rows_of(x, ix), self._rows_of(y, iy))
         elif _is_ct(x):
-            out = key.add_i64(self._rows_of(x, ix), iy)
+            out = _add_pt(key, self._rows_of(x, ix), iy)
         else:
-            out = key.add_i64(self._rows_of(y, iy), ix)
+            out = _add_pt(key, self._rows_of(y, iy), ix)
         return CiphertextArray(key, out, shape)
 
     def sub(self, x, y):
@@ -454,14 +492,9 @@ class Evaluator:
         if _is_ct(x) and _is_ct(y):  # evaluator.cc:66-68
             out = key.sub(self._rows_of(x, ix), self._rows_of(y, iy))
         elif _is_ct(x):  # Add(ct, -p), evaluator.cc:70-72
-            neg = [-int(v) for v in iy]
-            if min(iy, default=0) == np.iinfo(np.int64).min:
-                gm = key.encrypt(neg, np.zeros((len(neg), 1), np.uint32))
-                out = key.add(self._rows_of(x, ix), gm)
-            else:
-                out = key.add_i64(self._rows_of(x, ix), -iy)
+            out = _add_pt(key, self._rows_of(x, ix), _negated(iy))
         else:  # Add(Negate(ct), p), evaluator.cc:78-80
-            out = key.add_i64(key.negate(self._rows_of(y, iy)), ix)
+            out = _add_pt(key, key.negate(self._rows_of(y, iy)), ix)
         return CiphertextArray(key, out, shape)
 
     def mul(self, x, y):
@@ -474,7 +507,9 @@ class Evaluator:
         if ix.size == 0:
             return CiphertextArray(key, key._ct_out(0), shape)
         ct, idx, k = (x, ix, iy) if _is_ct(x) else (y, iy, ix)
-        return CiphertextArray(key, key.mul_i64(self._rows_of(ct, idx), k), shape)
+        rows = self._rows_of(ct, idx)
+        out = key.mul(rows, _ints(k)) if k.dtype == object else key.mul_i64(rows, k)
+        return CiphertextArray(key, out, shape)
 
     # ------------------------------------------------------------ products
     def matmul(self, x, y):
@@ -504,7 +539,12 @@ class Evaluator:
             w = _pt(x).reshape(n, k)[i, l].ravel()
             ct = y
         offs = np.arange(0, n * m * k + 1, k, dtype=np.int64)
-        out = key.dot_gather(ct.data, np.ascontiguousarray(idx, dtype=np.int64), np.ascontiguousarray(w), offs)
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        if w.dtype == object:  # weights beyond int64: the powers, then one segmented sum
+            terms = key.mul(self._rows_of(ct, idx), _ints(w)) if idx.size else key._ct_out(0)
+            out = key.sum(terms, offs)
+        else:
+            out = key.dot_gather(ct.data, idx, np.ascontiguousarray(w), offs)
         return CiphertextArray(key, out, out_shape)
 
     # ------------------------------------------------------------ reductions
@@ -590,7 +630,7 @@ class Evaluator:
     def _bucket_sum_plain(x, sm, om, B, cumsum):
         x2 = x.reshape(x.shape[0], -1)
         sub = np.nonzero(sm.ravel() > 0)[0]
-        out = np.zeros((B * om.shape[1], x2.shape[1]), dtype=np.int64)
+        out = np.zeros((B * om.shape[1], x2.shape[1]), dtype=x2.dtype)
         for f in range(om.shape[1]):
             for i in sub:
                 out[f * B + om[i, f]] += x2[i]
@@ -612,6 +652,11 @@ class PublicKey:
     @staticmethod
     def load_from(buf: bytes) -> "PublicKey":
         return PublicKey(*wire.get_public_key(bytes(buf)))
+
+    def plaintext_bound(self) -> int:
+        """n >> 1: the largest |m| an encryption takes
+        (paillier_zahlen/public_key.h:51-54, public_key.cc:30, encryptor.cc:38-40)."""
+        return self.n >> 1
 
     def __eq__(self, other) -> bool:
         return isinstance(other, PublicKey) and (self.n, self.h_s) == (other.n, other.h_s)
@@ -668,7 +713,8 @@ class HeKit:
         return SecretKey(self.key.p, self.key.q) if self.key.p is not None else None
 
     def array(self, ndarray) -> np.ndarray:
-        """A plaintext array (numpy int64)."""
+        """A plaintext array: numpy int64, or Python ints where a value needs
+        more than 64 bits."""
         return _pt(ndarray)
 
 
@@ -681,6 +727,10 @@ def setup(key_bits=2048, seed: int | None = None, ctx: E.Context | None = None,
     ctx = ctx or E.Context(0)
     if isinstance(key_bits, PublicKey):
         pk = key_bits
+        if isinstance(seed, SecretKey):  # setup(pk, sk), the reference's order
+            if secret_key is not None:
+                raise TypeError("setup: two secret keys given")
+            secret_key, seed = seed, None
         bits = (pk.n.bit_length() + 31) // 32 * 32
         p, q = (secret_key.p, secret_key.q) if secret_key is not None else (None, None)
         if p is not None and p * q != pk.n:

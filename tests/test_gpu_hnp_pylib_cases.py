@@ -3,8 +3,10 @@
 test_encrypt_with_audit, test_matmul, test_evaluate_parallel,
 test_ciphertext_slice_2d / _1d, test_batch_select_sum (at its size: a
 1,000,000 x 2 matrix and 50,000 selections of 150 rows),
-test_feature_wise_bucket_sum and test_batch_feature_wise_bucket_sum.  The
-inputs and expected values are the reference's; its random inputs are drawn
+test_feature_wise_bucket_sum and test_batch_feature_wise_bucket_sum; and of
+heu/pylib/phe_test.py's Paillier cases (test_enc_dec with the largest
+plaintext, test_add_inplace, key and ciphertext serialization) on 0-d arrays.
+The inputs and expected values are the reference's; its random inputs are drawn
 here from seeded generators.  Plaintext matrices are numpy arrays and phe
 scalars 0-d arrays.  Where the reference checks a plaintext-only path (matmul,
 batch_select_sum), the same inputs also run encrypted and decrypt to the same
@@ -201,3 +203,77 @@ def test_batch_feature_wise_bucket_sum(kit):
     assert first[1, 0] == 0 and first[1, 1] == 0
     assert second[0, 0] == 0 and second[0, 1] == 0
     assert second[1, 0] == m1[50:, 0].sum() and second[1, 1] == m1[50:, 1].sum()
+
+
+def test_phe_enc_dec(kit):
+    """phe_test.py:110-135 on 0-d arrays: ct+ct, ct-ct, ct+pt, ct-pt, and the
+    largest plaintext, n/2 - 1 (far past int128), minus 1."""
+    enc, dec, ev = kit.encryptor(), kit.decryptor(), kit.evaluator()
+    ct1, ct2 = enc.encrypt(np.array(123)), enc.encrypt(np.array(456))
+    print("pylib cases: phe test_enc_dec", flush=True)
+    assert int(dec.decrypt(ev.add(ct1, ct2))) == 123 + 456
+    assert int(dec.decrypt(ev.sub(ct1, ct2))) == 123 - 456
+    assert int(dec.decrypt(ev.sub(0, ct1))) == -123
+    assert int(dec.decrypt(ev.add(ct1, 100))) == 123 + 100
+    assert int(dec.decrypt(ev.sub(ct1, 100))) == 123 - 100
+    max_p = kit.public_key().plaintext_bound() - 1
+    assert max_p > 2**128
+    max_ct = ev.sub(enc.encrypt(np.array(max_p, dtype=object)), 1)
+    assert int(dec.decrypt(max_ct)) == max_p - 1
+    bound = kit.public_key().plaintext_bound()
+    assert int(dec.decrypt(enc.encrypt(np.array(-bound, dtype=object)))) == -bound
+    from heu_amd._native import HeuError
+
+    with pytest.raises(HeuError):
+        enc.encrypt(np.array(bound + 1, dtype=object))
+
+
+def test_phe_add_inplace(kit):
+    """phe_test.py:137-144: a running ciphertext sum of 0..99, plus 100."""
+    enc, dec, ev = kit.encryptor(), kit.decryptor(), kit.evaluator()
+    print("pylib cases: phe test_add_inplace", flush=True)
+    total = enc.encrypt(np.array(0))
+    for i in range(100):
+        total = ev.add(total, enc.encrypt(np.array(i)))
+    total = ev.add(total, 100)
+    assert int(dec.decrypt(total)) == 5050
+
+
+def test_phe_key_serialize(kit):
+    """phe_test.py:256-296: keys and ciphertexts leave through their wire forms
+    and a second kit built from the keys decrypts the first kit's ciphertexts."""
+    from heu_amd import engine, hnp
+
+    print("pylib cases: phe key serialize", flush=True)
+    pk2 = hnp.PublicKey.load_from(kit.public_key().serialize())
+    sk2 = hnp.SecretKey.load_from(kit.secret_key().serialize())
+    assert pk2 == kit.public_key() and sk2 == kit.secret_key()
+    kit2 = hnp.setup(pk2, sk2, ctx=engine.Context(0))
+    ct1 = kit.encryptor().encrypt(np.array(123))
+    ct2 = hnp.CiphertextArray.load_from(kit2, ct1.serialize())
+    assert int(kit2.decryptor().decrypt(ct2)) == 123
+    d = kit2.evaluator().sub(ct2, kit2.encryptor().encrypt(np.array(456)))
+    assert int(kit.decryptor().decrypt(hnp.CiphertextArray.load_from(kit, d.serialize()))) == 123 - 456
+
+
+def test_plaintexts_beyond_int64(kit):
+    """Plaintexts and scalars past int64 (BigInts in the reference) through
+    encrypt, ct+pt, ct-pt, ct x pt and matmul."""
+    enc, dec, ev = kit.encryptor(), kit.decryptor(), kit.evaluator()
+    rng = np.random.default_rng(7)
+    big = np.array([int(v) << 90 for v in rng.integers(-(2**20), 2**20, 12)], dtype=object).reshape(3, 4)
+    small = rng.integers(-1000, 1000, (3, 4))
+    print("pylib cases: plaintexts beyond int64", flush=True)
+    cb = enc.encrypt(big)
+    assert dec.decrypt(cb).tolist() == big.tolist()
+    cs = enc.encrypt(small)
+    assert dec.decrypt(ev.add(cs, big)).tolist() == (small.astype(object) + big).tolist()
+    assert dec.decrypt(ev.sub(cs, big)).tolist() == (small.astype(object) - big).tolist()
+    assert dec.decrypt(ev.sub(big, cs)).tolist() == (big - small.astype(object)).tolist()
+    scal = np.array([[1 << 100, -(1 << 70), 3, -1]], dtype=object)
+    assert dec.decrypt(ev.mul(cs, scal)).tolist() == (small.astype(object) * scal).tolist()
+    w = np.array([1 << 80, -(1 << 66), 7, 0], dtype=object)
+    assert dec.decrypt(ev.matmul(cs, w)).tolist() == (small.astype(object) @ w).tolist()
+    with_min = np.array([np.iinfo(np.int64).min, 5], np.int64)
+    two = enc.encrypt(np.array([3, 4]))
+    assert dec.decrypt(ev.sub(two, with_min)).tolist() == [3 - int(with_min[0]), -1]

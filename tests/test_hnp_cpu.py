@@ -157,3 +157,27 @@ def test_heu_key_rules_select_sum():
     assert ev.select_sum(m, ([0, 1], [0, 1])) == 0 + 1 + 30 + 31
     mask = m[:, 0] % 60 == 0
     assert ev.select_sum(m, (mask, 1)) == m[mask, 1].sum()
+
+
+def test_plaintexts_beyond_int64():
+    """Plaintexts are BigInts in the reference: values past int64 stay Python
+    ints through the plaintext paths; values that fit become int64."""
+    from heu_amd.hnp import Evaluator, _pt
+
+    ev = Evaluator(None)
+    big = np.array([2**100, -(2**70), 5], dtype=object)
+    a = _pt(big)
+    assert a.dtype == object and a.tolist() == [2**100, -(2**70), 5]
+    assert _pt(np.array([1, 2], dtype=object)).dtype == np.int64
+    assert _pt(np.array([2**64 - 1], dtype=np.uint64)).tolist() == [2**64 - 1]
+    assert ev.add(big, 1).tolist() == [2**100 + 1, -(2**70) + 1, 6]
+    assert ev.mul(big, big).tolist() == [2**200, 2**140, 25]
+    assert ev.sum(big) == 2**100 - 2**70 + 5
+    assert ev.matmul(big, big) == 2**200 + 2**140 + 25
+    x = np.array([[2**80], [1], [2**80]], dtype=object)
+    got = ev.feature_wise_bucket_sum(x, np.ones(3, np.int8), np.array([[0], [1], [0]], np.int8), 2)
+    assert got.tolist() == [[2**81], [1]]
+    with pytest.raises(TypeError):
+        _pt(np.array([1.5]))
+    with pytest.raises(TypeError):
+        _pt(np.array(["1"], dtype=object))
