@@ -67,10 +67,14 @@ class CiphertextArray:
     def size(self) -> int:
         return int(np.prod(self.shape, dtype=np.int64))
 
+    @property
     def rows(self) -> int:
+        """Rows of the array as the reference's matrix (bind_numpy.cc:62-63)."""
         return self.shape[0] if self.ndim else 1
 
+    @property
     def cols(self) -> int:
+        """Columns of the array as the reference's matrix (a vector has one)."""
         return self.shape[1] if self.ndim == 2 else 1
 
     def _flat_index(self, key) -> np.ndarray:
@@ -97,7 +101,7 @@ class CiphertextArray:
         if value.key is not self.key:
             raise ValueError("the assigned ciphertexts belong to another key")
         block = _block_of(self.shape, key)[0]
-        pr, pc = value.rows(), value.cols()
+        pr, pc = value.rows, value.cols
         if value.ndim == 1 and block.shape[0] == 1 and block.shape[1] > 1:
             pr, pc = pc, pr
         if (pr, pc) != block.shape:
@@ -136,7 +140,7 @@ class CiphertextArray:
     # ------------------------------------------------------------ wire format
     def _colmajor(self) -> np.ndarray:
         """Positions of `data` in Eigen's column-major order of (rows, cols)."""
-        return self._colmajor_of(self.rows(), self.cols())
+        return self._colmajor_of(self.rows, self.cols)
 
     def serialize(self, format: str = MatrixSerializeFormat.Best) -> bytes:
         """DenseMatrix<Ciphertext>::Serialize (numpy/matrix.h:221-265), or
@@ -152,7 +156,7 @@ class CiphertextArray:
         else:
             mont = key.to_mont(self.data, MONT_WORD_BITS).cpu().numpy().view(np.uint32)
             mont = mont[self._colmajor()]
-        return put[format](mont, self.rows(), self.cols(), self.ndim)
+        return put[format](mont, self.rows, self.cols, self.ndim)
 
     @staticmethod
     def load_from(key, buf: bytes, format: str = MatrixSerializeFormat.Best) -> "CiphertextArray":
@@ -583,7 +587,7 @@ class Evaluator:
     def feature_wise_bucket_sum(self, x, subgroup_map, order_map, bucket_num: int, cumsum: bool = False):
         """Per feature f, bucket b and column c: the sum of x[i, c] over the rows
         i of the subgroup with order_map[i, f] == b (cumulated over b with
-        cumsum); shape (bucket_num * features, x.cols())
+        cumsum); shape (bucket_num * features, x.cols)
         (numpy/evaluator.cc:378-466, extension_functions.cc:89-107)."""
         return self.batch_feature_wise_bucket_sum(x, [subgroup_map], order_map, bucket_num, cumsum)[0]
 
@@ -592,9 +596,9 @@ class Evaluator:
             return [self._bucket_sum_plain(_pt(x), np.asarray(sm), np.asarray(order_map), bucket_num, cumsum)
                     for sm in subgroup_maps]
         om = np.asarray(order_map)
-        if om.ndim != 2 or om.shape[0] != x.rows():
+        if om.ndim != 2 or om.shape[0] != x.rows:
             raise ValueError("order map and x should have the same number of rows")
-        rows, cols, feats, B = x.rows(), x.cols(), om.shape[1], int(bucket_num)
+        rows, cols, feats, B = x.rows, x.cols, om.shape[1], int(bucket_num)
         if om.size and (om.min() < 0 or om.max() >= B):
             raise ValueError("order map entries must be in [0, bucket_num)")
         idx_parts, lens = [], []

@@ -34,7 +34,7 @@ def test_evaluate(kit):
     pt1, pt2 = np.array([[1, 2], [3, 4]]), np.array([[4, 5], [6, 7]])
     ct1, ct2 = enc.encrypt(pt1), enc.encrypt(pt2)
     print("pylib cases: test_evaluate", flush=True)
-    assert (ct2.rows(), ct2.cols(), ct2.shape) == (2, 2, (2, 2))
+    assert (ct2.rows, ct2.cols, ct2.shape) == (2, 2, (2, 2))
     ans = np.array([[5, 7], [9, 11]])
     assert np.array_equal(ev.add(pt1, pt2), ans)
     for got in (ev.add(ct1, pt2), ev.add(pt1, ct2), ev.add(ct1, ct2)):

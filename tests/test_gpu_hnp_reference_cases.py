@@ -96,8 +96,8 @@ def test_select_sum_works(kit):
     plain_in = np.array([[0], [0], [1000000], [1000000], [0]], np.int64)
     encrypted_in = enc.encrypt(plain_in)
     encrypted_in2 = hnp.CiphertextArray.load_from(kit, encrypted_in.serialize())
-    assert encrypted_in2.rows() == encrypted_in.rows()
-    assert encrypted_in2.cols() == encrypted_in.cols()
+    assert encrypted_in2.rows == encrypted_in.rows
+    assert encrypted_in2.cols == encrypted_in.cols
     assert encrypted_in2.ndim == encrypted_in.ndim
     assert int(dec.decrypt(ev.select_sum(encrypted_in2, ([0, 1], [0])))) == 0
     assert int(dec.decrypt(ev.select_sum(encrypted_in2, ([0, 1, 2], [0])))) == 1000000
