@@ -722,12 +722,31 @@ class HeKit:
         return _pt(ndarray)
 
 
+# the ZPaillier aliases of phe/base/schema.cc:47-48, plus the names a HEU
+# build registers this engine's schema under
+_SCHEMA_ALIASES = ("z-paillier", "zpaillier", "paillier", "paillier_z", "paillier_zahlen",
+                   "hpaillier", "paillier_hip", "paillier_mi355x")
+
+
+def parse_schema(name: str) -> str:
+    """ParseSchemaType (phe/base/schema.cc:77-87), case-insensitive, for the one
+    schema this engine runs; any other schema raises ValueError."""
+    if name.lower() not in _SCHEMA_ALIASES:
+        raise ValueError(f"Unknown schema type {name}: this engine runs Paillier (z-paillier) only")
+    return "z-paillier"
+
+
 def setup(key_bits=2048, seed: int | None = None, ctx: E.Context | None = None,
           r_bits: int | None = None, window_bits: int = 0, secret_key: SecretKey | None = None) -> HeKit:
     """Generates a key (seeded or from the OS RNG) and makes it resident; or,
     given a PublicKey (and optionally its SecretKey) as the first argument,
     makes that key resident (hnp.setup(pk[, sk]) of bind_numpy.cc).
-    r_bits defaults to key_bits / 2 like the reference's GetRn (encryptor.cc:27)."""
+    r_bits defaults to key_bits / 2 like the reference's GetRn (encryptor.cc:27).
+    The reference's setup(schema_string[, key_size]) form is taken too:
+    setup("z-paillier", 2048), key size 2048 when none is given."""
+    if isinstance(key_bits, str):
+        parse_schema(key_bits)
+        key_bits, seed = (2048 if seed is None else int(seed)), None
     ctx = ctx or E.Context(0)
     if isinstance(key_bits, PublicKey):
         pk = key_bits

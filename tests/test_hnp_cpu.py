@@ -181,3 +181,15 @@ def test_plaintexts_beyond_int64():
         _pt(np.array([1.5]))
     with pytest.raises(TypeError):
         _pt(np.array(["1"], dtype=object))
+
+
+def test_schema_names():
+    """ParseSchemaType's ZPaillier aliases (phe/base/schema.cc:47-48, any case)
+    name this engine's schema; other schemas are refused."""
+    from heu_amd.hnp import parse_schema
+
+    for name in ("z-paillier", "ZPaillier", "paillier", "PAILLIER_ZAHLEN", "paillier_z", "hpaillier"):
+        assert parse_schema(name) == "z-paillier"
+    for name in ("ou", "f-paillier", "elgamal", "mock", ""):
+        with pytest.raises(ValueError):
+            parse_schema(name)
