@@ -19,6 +19,7 @@
 #include <openssl/bn.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -166,8 +167,8 @@ struct In {
   bool bigint(BIGNUM* r) {
     const uint8_t* b;
     size_t n;
-    if (!blob(&b, &n) || n < 1 || b[0] > 1) return ok = false;
-    BN_lebin2bn(b + 1, (int)(n - 1), r);
+    if (!blob(&b, &n) || n < 1 || b[0] > 1 || n - 1 > (size_t)INT_MAX) return ok = false;
+    if (!BN_lebin2bn(b + 1, (int)(n - 1), r)) return ok = false;
     BN_set_negative(r, b[0] == 1 && !BN_is_zero(r));
     return true;
   }
@@ -232,6 +233,13 @@ int to_words(const BIGNUM* v, uint32_t* w, uint32_t words) {
 }
 
 uint32_t words_of(const BIGNUM* v) { return (uint32_t)((BN_num_bits(v) + 31) / 32); }
+
+// rows x cols == count, without the product wrapping around
+bool shape_holds(int64_t rows, int64_t cols, uint64_t count) {
+  if (rows < 0 || cols < 0) return false;
+  if (cols != 0 && (uint64_t)rows > UINT64_MAX / (uint64_t)cols) return false;
+  return (uint64_t)rows * (uint64_t)cols == count;
+}
 
 // ---- interconnection form (numpy/matrix.cc:39-111) -------------------------
 // A protobuf DataExchangeProtocol (package org.interconnection.v2.runtime).
@@ -398,7 +406,7 @@ int heu_wire_get_matrix(const uint8_t* in, size_t len, int64_t* rows, int64_t* c
   uint32_t four = 0, count = 0;
   int64_t nr = 0, nc = 0, nd = 0;
   if (!r.array(&four) || four != 4 || !r.integer(&nr) || !r.integer(&nc) || !r.integer(&nd) || !r.array(&count) ||
-      nr < 0 || nc < 0 || nd < 0 || nd > 2 || (uint64_t)nr * (uint64_t)nc != count)
+      nd < 0 || nd > 2 || !shape_holds(nr, nc, count))
     return fail(HEU_EINVAL, "not a serialized ciphertext matrix: [rows, cols, ndim, items] expected");
   if (rows) *rows = nr;
   if (cols) *cols = nc;
@@ -486,7 +494,7 @@ int heu_wire_get_matrix_ic(const uint8_t* in, size_t len, int64_t* rows, int64_t
   const size_t nd = m.shape.size();
   const int64_t nr = nd > 0 ? m.shape[0] : 1, nc = nd > 1 ? m.shape[1] : 1;
   if (nd > 2 || nr < 0 || nc < 0) return fail(HEU_EINVAL, "a ciphertext matrix has at most 2 non-negative dims");
-  if ((uint64_t)nr * (uint64_t)nc != m.items.size())
+  if (!shape_holds(nr, nc, m.items.size()))
     return fail(HEU_EINVAL, "Pb: shape and len not match (%lld x %lld, %zu items)", (long long)nr, (long long)nc,
                 m.items.size());
   if (rows) *rows = nr;
@@ -571,6 +579,8 @@ int heu_wire_get_secret_key(const uint8_t* in, size_t len, uint32_t* p, uint32_t
   if (!r.array(&four) || four != 4 || !r.bigint(lam) || !r.bigint(mu) || !r.bigint(bp) || !r.bigint(bq) ||
       r.at != r.len)
     return fail(HEU_EINVAL, "not a serialized ZPaillier secret key: [lambda, mu, p, q] expected");
+  if (BN_is_negative(bp) || BN_is_negative(bq) || BN_cmp(bp, BN_value_one()) <= 0 || BN_cmp(bq, BN_value_one()) <= 0)
+    return fail(HEU_EKEY, "p and q must be integers above 1");
   // the packed lambda and mu must be those of p and q
   BN_CTX* ctx = BN_CTX_new();
   Bn n, l2, t, m2;

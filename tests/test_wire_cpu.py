@@ -225,3 +225,31 @@ def test_matrix_ic_rejects(wire, ic_pb):
             wire.get_matrix_ic(b, 128)
     with pytest.raises(HeuError):  # a 0-d matrix is 1 x 1
         wire.put_matrix_ic(np.zeros((2, 128), np.uint32), 2, 1, 0)
+
+
+def test_shapes_whose_product_wraps_are_refused(wire, ic_pb):
+    """rows x cols is compared with the item count without wrapping around:
+    2^32 x 2^32 is not 0 items."""
+    from heu_amd._native import HeuError
+
+    with pytest.raises(HeuError):
+        wire.get_matrix(msgpack.packb([2**32, 2**32, 2, []], use_bin_type=False), 128)
+    with pytest.raises(HeuError):
+        wire.get_matrix(msgpack.packb([2**62, 4, 2, []], use_bin_type=False), 128)
+    with pytest.raises(HeuError):
+        wire.get_matrix_ic(ic_message(ic_pb, 2**32, 2**32, 2, []).SerializeToString(), 128)
+    # an empty matrix with one zero dimension is still fine
+    assert wire.matrix_shape(msgpack.packb([0, 2**40, 2, []], use_bin_type=False)) == (0, 2**40, 2)
+
+
+def test_secret_key_primes_must_exceed_one(wire):
+    """Primes of a serialized secret key must be integers above 1, whatever the
+    lambda and mu beside them."""
+    from heu_amd._native import HeuError
+
+    for p, q in ((-7, -11), (1, 7), (0, 5)):
+        lam = (p - 1) * (q - 1) // 2
+        mu = 0
+        sk = msgpack.packb([bigint_bytes(x) for x in (lam, mu, p, q)], use_bin_type=False) + IDX
+        with pytest.raises(HeuError):
+            wire.get_secret_key(sk)
