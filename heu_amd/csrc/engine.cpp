@@ -649,10 +649,9 @@ int heu_key_create(heu_ctx* ctx, const uint32_t* n_w, const uint32_t* hs_w, uint
     key->nhalf_host = bn_to_words(h, 2 * ops->p_words);
   }
   // fixed-base table; window 0 = the widest window whose table fits the
-  // per-key budget (HEU_TABLE_BYTES, default 1 GiB), at least the reference's 10
+  // per-key budget (table_budget_bytes), at least the reference's 10
   if (window_bits == 0) {
-    size_t budget = (size_t)1 << 30;
-    if (const char* b = getenv("HEU_TABLE_BYTES")) budget = (size_t)strtoull(b, nullptr, 10);
+    const size_t budget = table_budget_bytes();
     window_bits = 10;
     for (uint32_t w = 16; w > 10; --w) {
       const size_t bytes = ((size_t)((r_bits + w - 1) / w) << w) * (size_t)key->nl_n * 4;

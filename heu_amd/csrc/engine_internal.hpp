@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -15,6 +16,16 @@
 #include "kernels.hpp"
 
 namespace heu_engine {
+
+// Per-key budget of the fixed-base h_s table: HEU_TABLE_BYTES, default 4 GiB.
+// At 4 GiB a 2048-bit key takes 16-bit windows for r of 1,024 bits (2.4 GB)
+// as well as for r of 256 bits (0.6 GB); at 1 GiB the first fell to 14 bits,
+// with 15% slower encryption.
+inline size_t table_budget_bytes() {
+  size_t budget = (size_t)4 << 30;
+  if (const char* b = std::getenv("HEU_TABLE_BYTES")) budget = (size_t)std::strtoull(b, nullptr, 10);
+  return budget;
+}
 
 // Records the message of a failing call (heu_last_error) and returns `code`.
 int fail(int code, const char* fmt, ...);

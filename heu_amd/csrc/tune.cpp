@@ -3,7 +3,7 @@
 // For one key it times, on synthetic inputs and on the device at hand, the
 // candidates of three parameters and keeps the fastest of each:
 //   - the fixed-base table window (encryption throughput; candidates whose
-//     table fits HEU_TABLE_BYTES, default 1 GiB);
+//     table fits HEU_TABLE_BYTES, default 4 GiB);
 //   - the lanes per bignum of the CRT decryption (decryption throughput);
 //   - the grid multiplier of the persistent work-list launches, i.e. how many
 //     times the resident workgroups per CU a launch starts (occupancy), for the
@@ -97,8 +97,7 @@ extern "C" int heu_autotune(heu_key* key, uint32_t what, char* report, size_t re
   std::string js = "{\"key_bits\": " + std::to_string(bits);
 
   if (what & HEU_TUNE_TABLE) {
-    size_t budget = (size_t)1 << 30;
-    if (const char* b = getenv("HEU_TABLE_BYTES")) budget = (size_t)strtoull(b, nullptr, 10);
+    const size_t budget = table_budget_bytes();
     uint32_t best_w = (uint32_t)key->unit;
     double best_t = 1e30;
     js += ", \"table_window\": {\"elements\": " + std::to_string(n_enc) + ", \"ms\": {";

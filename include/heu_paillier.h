@@ -83,7 +83,7 @@ int heu_keygen(uint32_t key_bits, const uint64_t *seed, uint32_t *n, uint32_t *h
  * randomness r the fixed-base table must cover (the reference draws
  * key_bits/2 bits, encryptor.cc:27); window_bits: table window (the
  * reference's kExpUnitBits = 10, public_key.cc:21), or 0 for the widest window
- * (<= 16) whose table fits HEU_TABLE_BYTES (default 1 GiB) per key. */
+ * (<= 16) whose table fits HEU_TABLE_BYTES (default 4 GiB) per key. */
 int heu_key_create(heu_ctx *ctx, const uint32_t *n, const uint32_t *h_s, uint32_t n_words, const uint32_t *p,
                    const uint32_t *q, uint32_t prime_words, uint32_t r_bits, uint32_t window_bits, heu_key **out);
 int heu_key_destroy(heu_key *key);
