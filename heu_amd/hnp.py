@@ -598,7 +598,7 @@ class Evaluator:
         om = np.asarray(order_map)
         if om.ndim != 2 or om.shape[0] != x.rows:
             raise ValueError("order map and x should have the same number of rows")
-        rows, cols, feats, B = x.rows, x.cols, om.shape[1], int(bucket_num)
+        cols, feats, B = x.cols, om.shape[1], int(bucket_num)
         if om.size and (om.min() < 0 or om.max() >= B):
             raise ValueError("order map entries must be in [0, bucket_num)")
         idx_parts, lens = [], []
