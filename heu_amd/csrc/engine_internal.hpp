@@ -223,8 +223,8 @@ struct heu_key {
   struct Half {
     heu_engine::DevBuf m, one, r2, r3;
     uint32_t mprime = 0;
-    ModArgs args() const {
-      return ModArgs{m.as<uint32_t>(), one.as<uint32_t>(), r2.as<uint32_t>(), r3.as<uint32_t>(), mprime};
+    ModArgs args() const {  // no R-power table: only sums mod n^2 read one
+      return ModArgs{m.as<uint32_t>(), one.as<uint32_t>(), r2.as<uint32_t>(), r3.as<uint32_t>(), mprime, nullptr};
     }
   };
   Half p2[heu_hip::kDecShapes], q2[heu_hip::kDecShapes];
