@@ -69,6 +69,30 @@ def test_decrypt_shapes_and_table_windows(eng, oracle_mod, ctx, bits):
     key.close()
 
 
+def test_default_window_follows_table_budget(eng, oracle_mod, ctx, monkeypatch):
+    """Without a window, a key takes the widest (10..16 bits) whose table of
+    ceil(r_bits / w) * 2^w entries of 576 B fits HEU_TABLE_BYTES (default
+    4 GiB); encryption gives the oracle's bytes at each."""
+    kd = eng.keygen(2048, 72)
+    ok = oracle_mod.Key(kd["n"], kd["hs"], kd["p"], kd["q"])
+    m = np.arange(-100, 100, dtype=np.int64)
+    rng = np.random.default_rng(72)
+    cases = [(256, 64 << 20, 12), (256, None, 16), (1024, None, 16), (1024, 1 << 30, 14), (256, 1 << 20, 10)]
+    for r_bits, budget, want in cases:
+        if budget is None:
+            monkeypatch.delenv("HEU_TABLE_BYTES", raising=False)
+        else:
+            monkeypatch.setenv("HEU_TABLE_BYTES", str(budget))
+        print(f"table budget {budget}, r_bits {r_bits}", flush=True)
+        key = eng.Key.from_keygen(ctx, kd, r_bits=r_bits, window_bits=0)
+        assert key.table_window == want, (r_bits, budget)
+        r = rng.integers(0, 2**32, size=(m.size, r_bits // 32), dtype=np.uint64).astype(np.uint32)
+        ct = key.encrypt_i64(m, r)
+        assert np.array_equal(host(ct), ok.encrypt_i64(m, r))
+        key.close()
+    monkeypatch.delenv("HEU_TABLE_BYTES", raising=False)
+
+
 def test_autotune_keeps_results(eng, oracle_mod, ctx):
     kd = eng.keygen(2048, 70)
     key = eng.Key.from_keygen(ctx, kd, r_bits=256, window_bits=10)
