@@ -534,7 +534,7 @@ def main():
         line["cpu_baseline"]["speedup_gpu_over_cpu"] = value / cpu["value"]
         sp = {op: per_op_rate[op] / cpu["per_op"][op] for op in ops}
         line["cpu_baseline"]["speedup_per_op"] = sp
-        # the north star's bar: >= 50x the all-core host rate for encrypt and decrypt
+        # the target: >= 50x the all-core host rate for encrypt and decrypt
         line["cpu_baseline"]["target_50x"] = {op: sp[op] >= 50.0 for op in ("enc", "dec")}
         line["cpu_baseline"]["speedup_per_core_per_op"] = {
             op: per_op_rate[op] / cpu["single_core_per_op"][op] for op in ops}
@@ -585,7 +585,7 @@ def main():
         pb = next(x for x in extra["5"]["plain_batched"] if x["offered_req_per_s"] == pick["offered_req_per_s"])
         line["plain_batched_p99_ms"] = pb["p99_ms"]
         line["plain_batched_batch_ms_mean"] = pb["batch_ms_mean"]
-        # the north star's serving bar: served p99 <= 2x the plain batched path's per-batch time
+        # the serving target: served p99 <= 2x the plain batched path's per-batch time
         # per-batch time as the mean and as the p99 of the plain batches (their
         # sizes span five orders of magnitude, so the two differ by 10-20x)
         for stat in ("mean", "p99"):

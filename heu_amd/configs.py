@@ -413,7 +413,7 @@ def run_config5(ctx: E.Context, seed: int = 0, rates=(15.0, 30.0, 45.0, 60.0, 90
         print(f"[config5] {rate:g} req/s: served", flush=True)
         lat_s, wall_s, tickets = _served(srv, pools, rq, arrivals)
         row = _summary(rate, lat_s, wall_s)
-        # the north star's bar: served p99 <= 2x the plain batched path's per-batch time
+        # the serving target: served p99 <= 2x the plain batched path's per-batch time
         row["p99_over_2x_plain_batch_mean"] = row["p99_ms"] / (2.0 * batched[-1]["batch_ms_mean"])
         row["p99_over_2x_plain_batch_p99"] = row["p99_ms"] / (2.0 * batched[-1]["batch_ms_p99"])
         served.append(row)
