@@ -2,9 +2,10 @@
 // HIP device.  The first cases restate the reference's own unit tests,
 // heu/library/algorithms/paillier_zahlen/paillier_test.cc:26-233
 // (CiphertextEvaluate, MinMaxDecrypt, PlaintextEvaluate1/2, BigNumberTest
-// EncDec/SubTest), through the single-element forms; the rest exercise the
-// span (vectorized) forms against closed forms computed with OpenSSL on the
-// host.  Prints one line per case and "ALL PASSED" at the end; exit status 0
+// EncDec/SubTest), through the single-element forms, and the reference's
+// tests of its CUDA backend's vectorized SPI (paillier_gpu/paillier_gpu_test.cc)
+// through every span overload; the rest exercise the span (vectorized) forms
+// against closed forms computed with OpenSSL on the host.  Prints one line per case and "ALL PASSED" at the end; exit status 0
 // iff every check held.
 #include <openssl/bn.h>
 
@@ -388,6 +389,115 @@ void VectorizedClosedForms(const Keys& k, size_t count) {
   EXPECT_THROW(evaluator.Add(cs(pc), cs(shorter)));
 }
 
+// The reference's tests of its CUDA backend's vectorized SPI,
+// heu/library/algorithms/paillier_gpu/paillier_gpu_test.cc:117-763, restated
+// with their values against the span forms: every Add / Sub / Mul overload
+// (ct, ct / ct, pt / pt, ct / pt, pt), the in-place forms, Negate and
+// Randomize.  Each test's value rows run as one span call.
+void GpuBackendCases(const Keys& k) {
+  Encryptor enc(k.pk);
+  Evaluator ev(k.pk);
+  Decryptor dec(k.pk, k.sk);
+  const BigInt lo(std::numeric_limits<int64_t>::lowest()), hi(std::numeric_limits<int64_t>::max());
+  using V = std::vector<BigInt>;
+  auto E = [&](const V& v) { return enc.Encrypt(cs(cptrs(v))); };
+  auto D = [&](const std::vector<Ciphertext>& c) { return dec.Decrypt(cs(cptrs(c))); };
+  auto same = [](const V& a, const V& b) { return a == b; };
+
+  // EncDecTest (:117-125)
+  V pairs{1, 2, -1, 0, -3, hi, -65530, -123, lo, 0};
+  EXPECT_TRUE(same(D(E(pairs)), pairs));
+  // CiphertextsAddTest (:157-165)
+  V a{-2, 123, -123, 234, lo, lo, hi}, b{-4, 234, 234, -123, hi, lo, hi};
+  V c{-6, 357, 111, 111, -1, lo + lo, hi + hi};
+  EXPECT_TRUE(same(D(ev.Add(cs(cptrs(E(a))), cs(cptrs(E(b))))), c));
+  // CiphertextsAddInplaceTest (:192-197)
+  {
+    V x{5, 5, -5, -5}, y{2, -7, 7, -7}, z{7, -2, 2, -12};
+    auto cx = E(x), cy = E(y);
+    auto mx = mptrs(cx);
+    ev.AddInplace(mspan(mx), cs(cptrs(cy)));
+    EXPECT_TRUE(same(D(cx), z));
+  }
+  // CiphertextsRandomize (:217-243): same plaintext, another ciphertext
+  {
+    V x{5, 0, -123, lo, hi};
+    auto cx = E(x);
+    const auto before = cx;
+    auto mx = mptrs(cx);
+    ev.Randomize(mspan(mx));
+    EXPECT_TRUE(same(D(cx), x));
+    bool all_changed = true;
+    for (size_t i = 0; i < cx.size(); ++i) all_changed = all_changed && cx[i] != before[i];
+    EXPECT_TRUE(all_changed);
+  }
+  // PlaintextsAddTest / PlaintextsAddInplace (:245-302)
+  {
+    V x{-5, 7, 3, 7, -3, -7}, y{8, -9, 8, 9, -8, -9}, z{3, -2, 11, 16, -11, -16};
+    EXPECT_TRUE(same(ev.Add(cs(cptrs(x)), cs(cptrs(y))), z));
+    auto mx = mptrs(x);
+    ev.AddInplace(mspan(mx), cs(cptrs(y)));
+    EXPECT_TRUE(same(x, z));
+  }
+  // CiphertextsAddPlaintexts, PlaintextsAddCiphertexts(Inplace) (:304-401)
+  {
+    V x{123, 123, -123, -123, lo, hi}, y{2, -2, 2, -2, hi, -2}, z{125, 121, -121, -125, -1, hi - BigInt(2)};
+    auto cx = E(x);
+    EXPECT_TRUE(same(D(ev.Add(cs(cptrs(cx)), cs(cptrs(y)))), z));
+    EXPECT_TRUE(same(D(ev.Add(cs(cptrs(y)), cs(cptrs(cx)))), z));
+    auto mx = mptrs(cx);
+    ev.AddInplace(mspan(mx), cs(cptrs(y)));
+    EXPECT_TRUE(same(D(cx), z));
+  }
+  // CtsMulPtsTest, PtsMulCtsTest, CtsMulPtsInplaceTest (:403-508)
+  {
+    V x{2, -2, 2, -2}, y{4, 4, -4, -4}, z{8, -8, -8, 8};
+    auto cx = E(x);
+    EXPECT_TRUE(same(D(ev.Mul(cs(cptrs(cx)), cs(cptrs(y)))), z));
+    EXPECT_TRUE(same(D(ev.Mul(cs(cptrs(y)), cs(cptrs(cx)))), z));
+    auto mx = mptrs(cx);
+    ev.MulInplace(mspan(mx), cs(cptrs(y)));
+    EXPECT_TRUE(same(D(cx), z));
+  }
+  // PtsMulPtsTest, PtsMulPtsInplaceTest (:458-533)
+  {
+    V x{5, 7, -5, 7, -5, -7}, y{8, 9, 8, -9, -8, -9}, z{40, 63, -40, -63, 40, 63};
+    EXPECT_TRUE(same(ev.Mul(cs(cptrs(x)), cs(cptrs(y))), z));
+    auto mx = mptrs(x);
+    ev.MulInplace(mspan(mx), cs(cptrs(y)));
+    EXPECT_TRUE(same(x, z));
+  }
+  // CiphertextsNegateTest, CiphertextsNegateInplaceTest (:535-593)
+  {
+    V x{5, -123, 0, hi}, z{-5, 123, 0, -hi};
+    auto cx = E(x);
+    EXPECT_TRUE(same(D(ev.Negate(cs(cptrs(cx)))), z));
+    auto mx = mptrs(cx);
+    ev.NegateInplace(mspan(mx));
+    EXPECT_TRUE(same(D(cx), z));
+  }
+  // CiphertextsSub, CtsSubPts, PtsSubCts, PtsSubPts and their in-place forms
+  // (:595-763): a - b with a, b in {(123, -7), (-123, -7), (123, 7)}, plus
+  // (iLow, -1) for ct - ct
+  {
+    V x{123, -123, 123, lo}, y{-7, -7, 7, -1}, z{130, -116, 116, lo + BigInt(1)};
+    auto cx = E(x), cy = E(y);
+    EXPECT_TRUE(same(D(ev.Sub(cs(cptrs(cx)), cs(cptrs(cy)))), z));
+    EXPECT_TRUE(same(D(ev.Sub(cs(cptrs(cx)), cs(cptrs(y)))), z));
+    EXPECT_TRUE(same(D(ev.Sub(cs(cptrs(x)), cs(cptrs(cy)))), z));
+    EXPECT_TRUE(same(ev.Sub(cs(cptrs(x)), cs(cptrs(y))), z));
+    auto cx2 = cx;
+    auto m1 = mptrs(cx), m2 = mptrs(cx2);
+    ev.SubInplace(mspan(m1), cs(cptrs(cy)));
+    ev.SubInplace(mspan(m2), cs(cptrs(y)));
+    EXPECT_TRUE(same(D(cx), z));
+    EXPECT_TRUE(same(D(cx2), z));
+    auto mx = mptrs(x);
+    ev.SubInplace(mspan(mx), cs(cptrs(y)));
+    EXPECT_TRUE(same(x, z));
+  }
+}
+
 // Spans long enough that the SPI splits its host work (packing, checks,
 // conversions, result objects) over threads: every path still gives the
 // closed forms, checked through decryption, and an error names the first bad
@@ -667,6 +777,7 @@ int main() {
   run("KeyGenerator.Generate(3072)", [&] { KeyGenerator::Generate(3072, &k3072.sk, &k3072.pk); });
   run("ZPaillierTest.MinMaxDecrypt(3072)", [&] { MinMaxDecrypt(k3072); });
   run("Vectorized.ClosedForms(3072, 65)", [&] { VectorizedClosedForms(k3072, 65); });
+  run("GPUTest cases of paillier_gpu_test.cc (2048)", [&] { GpuBackendCases(k2048); });
   run("Vectorized.SpansSplitOverThreads(2048, 50000)", [&] { SpansSplitOverThreads(seeded); });
   run("Concurrent chunks coalesce (2048 + 3072 keys)", [&] { ConcurrentChunks(seeded, k3072); });
   run("Vectorized.SpanRates(2048, 65536)", [&] { SpanRates(seeded, 65536); });
