@@ -390,7 +390,8 @@ void VectorizedClosedForms(const Keys& k, size_t count) {
 }
 
 // The reference's tests of its CUDA backend's vectorized SPI,
-// heu/library/algorithms/paillier_gpu/paillier_gpu_test.cc:117-763, restated
+// heu/library/algorithms/paillier_gpu/paillier_gpu_test.cc:117-763 and
+// paillier_gpu_test_con.cc:53-105, restated
 // with their values against the span forms: every Add / Sub / Mul overload
 // (ct, ct / ct, pt / pt, ct / pt, pt), the in-place forms, Negate and
 // Randomize.  Each test's value rows run as one span call.
@@ -407,6 +408,13 @@ void GpuBackendCases(const Keys& k) {
   // EncDecTest (:117-125)
   V pairs{1, 2, -1, 0, -3, hi, -65530, -123, lo, 0};
   EXPECT_TRUE(same(D(E(pairs)), pairs));
+  // paillier_gpu_test_con.cc: EncDecBigintTest (:53-73) up to the plaintext
+  // bound, and EncDecLongTest (:75-105), 100,000 elements of 33
+  const BigInt top = k.pk.n_half_ - BigInt(1);
+  V big{0, 1, 100, lo, top, -top};
+  EXPECT_TRUE(same(D(E(big)), big));
+  V many(100000, BigInt(33));
+  EXPECT_TRUE(same(D(E(many)), many));
   // CiphertextsAddTest (:157-165)
   V a{-2, 123, -123, 234, lo, lo, hi}, b{-4, 234, 234, -123, hi, lo, hi};
   V c{-6, 357, 111, 111, -1, lo + lo, hi + hi};
