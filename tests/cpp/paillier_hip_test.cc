@@ -4,7 +4,8 @@
 // (CiphertextEvaluate, MinMaxDecrypt, PlaintextEvaluate1/2, BigNumberTest
 // EncDec/SubTest), through the single-element forms, and the reference's
 // tests of its CUDA backend's vectorized SPI (paillier_gpu/paillier_gpu_test.cc)
-// through every span overload; the rest exercise the span (vectorized) forms
+// through every span overload, and the phe layer's evaluator and encryptor
+// tests on this schema; the rest exercise the span (vectorized) forms
 // against closed forms computed with OpenSSL on the host.  Prints one line per case and "ALL PASSED" at the end; exit status 0
 // iff every check held.
 #include <openssl/bn.h>
@@ -506,6 +507,81 @@ void GpuBackendCases(const Keys& k) {
   }
 }
 
+// The phe layer's tests on the ZPaillier schema, through the single-element
+// forms: heu/library/phe/evaluator_test.cc:31-140 (Evaluate, EvaluateInplace)
+// and encryptor_test.cc:32-90 (EncryptZero, MinMaxEnc).  Plaintexts are what
+// PlainEncoder makes of them: Encode(x) = x * scale, with scale 1 (`edr`) or
+// INT64_MAX (`encoder`, beyond int64).
+void PheCases(const Keys& k) {
+  Encryptor enc(k.pk);
+  Evaluator ev(k.pk);
+  Decryptor dec(k.pk, k.sk);
+  // Evaluate
+  Ciphertext ct0 = enc.Encrypt(BigInt(-12345));
+  ev.Randomize(&ct0);
+  EXPECT_EQ(dec.Decrypt(ct0), BigInt(-12345));
+  EXPECT_EQ(dec.Decrypt(ev.Negate(ct0)), BigInt(12345));
+  EXPECT_EQ(dec.Decrypt(ev.Add(ct0, ct0)), BigInt(-12345 * 2));
+  EXPECT_EQ(dec.Decrypt(ev.Add(ct0, BigInt(345))), BigInt(-12345 + 345));
+  const BigInt scale(std::numeric_limits<int64_t>::max());
+  const BigInt imax = scale;
+  const Ciphertext ct_max = enc.Encrypt((imax - BigInt(1)) * scale);
+  EXPECT_EQ(dec.Decrypt(ev.Add(ct_max, BigInt(1) * scale)), imax * scale);
+  EXPECT_EQ(dec.Decrypt(ev.Sub(ct0, ct0)), BigInt(0));
+  EXPECT_EQ(dec.Decrypt(ev.Sub(ct0, BigInt(345))), BigInt(-12345 - 345));
+  EXPECT_EQ(dec.Decrypt(ev.Sub(BigInt(789), ct0)), BigInt(12345 + 789));
+  const BigInt umax = imax * BigInt(2) + BigInt(1);  // UINT64_MAX
+  const Ciphertext ct_umax = enc.Encrypt(umax * scale);
+  EXPECT_EQ(dec.Decrypt(ev.Sub(ct_umax, BigInt(1) * scale)), (umax - BigInt(1)) * scale);
+  for (int64_t m : {0, 1, 2, 10}) EXPECT_EQ(dec.Decrypt(ev.Mul(ct0, BigInt(m))), BigInt(-12345 * m));
+  // EvaluateInplace
+  BigInt pt0(9876), pt1(1234);
+  Ciphertext c0 = enc.Encrypt(pt0), c1 = enc.Encrypt(pt1);
+  ev.Randomize(&c0);
+  EXPECT_EQ(dec.Decrypt(c0), BigInt(9876));
+  ev.NegateInplace(&c1);
+  pt1 = -pt1;
+  EXPECT_EQ(dec.Decrypt(c1), pt1);
+  ev.AddInplace(&c0, c1);
+  pt0 = pt0 + pt1;
+  EXPECT_EQ(dec.Decrypt(c0), pt0);
+  ev.AddInplace(&c0, BigInt(963));
+  pt0 = pt0 + BigInt(963);
+  EXPECT_EQ(dec.Decrypt(c0), pt0);
+  ev.AddInplace(&c0, BigInt(-741));
+  pt0 = pt0 + BigInt(-741);
+  EXPECT_EQ(dec.Decrypt(c0), pt0);
+  ev.SubInplace(&c0, c1);
+  pt0 = pt0 - pt1;
+  EXPECT_EQ(dec.Decrypt(c0), pt0);
+  ev.SubInplace(&c0, BigInt(852));
+  pt0 = pt0 - BigInt(852);
+  EXPECT_EQ(dec.Decrypt(c0), pt0);
+  ev.MulInplace(&c0, BigInt(10));
+  pt0 = pt0 * BigInt(10);
+  EXPECT_EQ(dec.Decrypt(c0), pt0);
+  // EncryptZero
+  Ciphertext z = enc.EncryptZero();
+  EXPECT_EQ(dec.Decrypt(z), BigInt(0));
+  ev.AddInplace(&z, enc.EncryptZero());
+  EXPECT_EQ(dec.Decrypt(z), BigInt(0));
+  ev.SubInplace(&z, enc.EncryptZero());
+  EXPECT_EQ(dec.Decrypt(z), BigInt(0));
+  EXPECT_EQ(dec.Decrypt(ev.Sub(BigInt(123), z)), BigInt(123));
+  for (int64_t m : {0, 123456, -123456}) {
+    ev.MulInplace(&z, BigInt(m));
+    EXPECT_EQ(dec.Decrypt(z), BigInt(0));
+  }
+  ev.NegateInplace(&z);
+  EXPECT_EQ(dec.Decrypt(z), BigInt(0));
+  // MinMaxEnc: |m| <= PlaintextBound (n / 2) encrypts, one past it throws
+  const BigInt bound = k.pk.n_half_;
+  EXPECT_THROW(enc.Encrypt(bound + BigInt(1)));
+  EXPECT_THROW(enc.Encrypt(-(bound + BigInt(1))));
+  EXPECT_EQ(dec.Decrypt(enc.Encrypt(bound)), bound);
+  EXPECT_EQ(dec.Decrypt(enc.Encrypt(-bound)), -bound);
+}
+
 // Spans long enough that the SPI splits its host work (packing, checks,
 // conversions, result objects) over threads: every path still gives the
 // closed forms, checked through decryption, and an error names the first bad
@@ -786,6 +862,7 @@ int main() {
   run("ZPaillierTest.MinMaxDecrypt(3072)", [&] { MinMaxDecrypt(k3072); });
   run("Vectorized.ClosedForms(3072, 65)", [&] { VectorizedClosedForms(k3072, 65); });
   run("GPUTest cases of paillier_gpu_test.cc (2048)", [&] { GpuBackendCases(k2048); });
+  run("phe EvaluatorTest / EncryptorTest cases (2048)", [&] { PheCases(k2048); });
   run("Vectorized.SpansSplitOverThreads(2048, 50000)", [&] { SpansSplitOverThreads(seeded); });
   run("Concurrent chunks coalesce (2048 + 3072 keys)", [&] { ConcurrentChunks(seeded, k3072); });
   run("Vectorized.SpanRates(2048, 65536)", [&] { SpanRates(seeded, 65536); });
