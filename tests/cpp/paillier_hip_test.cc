@@ -831,6 +831,20 @@ void Serialization(const Keys& k) {
   }
   last.pop_back();
   EXPECT_THROW(Ciphertext::Deserialize(pk2, last));
+  // phe/phe_test/ser_test.cc:31-108: keys and ciphertexts print with content,
+  // and a party holding only the public key's bytes evaluates on a client's
+  // ciphertext (AddInplace, Randomize) and sends it back
+  EXPECT_TRUE(k.pk.ToString().size() > 30 && k.sk.ToString().size() > 30);
+  const Ciphertext client = enc.Encrypt(BigInt(-963258741));
+  EXPECT_TRUE(client.ToString().size() >= 10);
+  PublicKey server_pk;
+  server_pk.Deserialize(k.pk.Serialize());
+  Evaluator server(server_pk);
+  Ciphertext at_server = Ciphertext::Deserialize(server_pk, client.Serialize(k.pk));
+  server.AddInplace(&at_server, BigInt(666));
+  server.Randomize(&at_server);
+  const Ciphertext back = Ciphertext::Deserialize(k.pk, at_server.Serialize(server_pk));
+  EXPECT_TRUE(dec2.Decrypt(back) == BigInt(-963258741 + 666));
   const auto audit = enc.EncryptWithAudit(BigInt(-77));
   const std::string want_c = "c:" + mod_mul(ct_value(audit.first), R, n2).ToHexString();
   EXPECT_TRUE(audit.second.rfind("p:-4D,rn:", 0) == 0);
